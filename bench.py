@@ -1,0 +1,186 @@
+"""Benchmark: Heston path-steps/s (1,048,576 paths x 252 steps), FP32 fused
+form vs the plain FP64 form, error vs the FP64 oracle; LSMC ms.
+
+A "step" is one pricing pass of the hot path: simulate 1,048,576 Heston
+full-truncation Euler paths of 252 steps on their block-split PCG64DXSM
+streams and price European calls at K = 90, 100, 110 in-register (no path
+storage).  `value` is the FP32 form's path-steps/s over the timed steps.
+
+Printed as one JSON line (rank 0, one GPU).  Extra keys carry the FP64 form,
+the per-strike error against the FP64 CPU oracle, Merton, LSMC timings, the
+roofline of the dominant kernel (timed with HIP events on the stream it runs
+on) and the CPU baseline (the C oracle with OpenMP on the host cores, on the
+same workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Heston path-steps/s (1M×252) FP32 vs FP64 GPU at ≤1e-4 rel price err; "
+          "LSMC ms")
+N_PATHS = 1_048_576
+STEPS = 252
+STRIKES = [90.0, 100.0, 110.0]
+HESTON = dict(s0=100.0, v0=0.04, mu=0.05, kappa=2.0, theta=0.04, xi=0.3, rho=-0.7)
+MERTON = dict(s0=100.0, mu=0.05, sigma=0.20, lambda_=1.0, mu_j=-0.05, sigma_j=0.10)
+R = 0.05
+T = 1.0
+SEED = 42
+
+# MI355X published peaks
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12  # FP64 / non-packed FP32 lane-ops (39.3 T/s)
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic VALU work per Heston FP32 path-step (lane-instructions of the
+# minimal code path, counted from the kernel's gfx950 ISA):
+#   2.06 u64 draws x 30 (PCG64DXSM: DXSM output 13 + 128x64-bit LCG step 17)
+#   2 normals x 10 (ziggurat fast path: layer/threshold test, u32->f32, scale, sign)
+#   1 stepper x 14 (FP32 full-truncation Euler in log space: 6 FMA, sqrt, max, ...)
+OPS_PER_STEP_HESTON32 = 2.06 * 30 + 2 * 10 + 14
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def time_calls(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    _sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def kernel_ms(S, fn, reps=3):
+    """Average duration of the dominant kernel (HIP events on its stream)."""
+    S.enable_kernel_timing(True)
+    tot = 0.0
+    for _ in range(reps):
+        fn()
+        ms, _ = S.last_kernel_ms()
+        tot += ms
+    S.enable_kernel_timing(False)
+    return tot / reps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip Merton and LSMC")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build "
+              "has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import stocha_amd as S
+
+    def heston(prec):
+        return S.european_price("heston", HESTON, T, STEPS, N_PATHS, STRIKES, r=R, seed=SEED,
+                                precision=prec)
+
+    # ---- timed: FP32 fused (the headline) ----
+    t32 = time_calls(lambda: heston(32), args.steps, args.warmup)
+    k32 = kernel_ms(S, lambda: heston(32))
+    r32 = heston(32)
+    # ---- FP64 plain form (reference arithmetic), same paths ----
+    t64 = time_calls(lambda: heston(64), max(2, args.steps // 2), 1)
+    k64 = kernel_ms(S, lambda: heston(64))
+    r64 = heston(64)
+
+    ps = N_PATHS * STEPS
+    value = ps / t32
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded PCG64DXSM block-split streams, seed 42)",
+        "config": {"workload": "heston_full_truncation_euler_fused_european_call",
+                   "n_paths": N_PATHS, "path_steps": STEPS, "strikes": STRIKES,
+                   "params": HESTON, "r": R, "t": T, "seed": SEED,
+                   "payoff_sums": "fp64"},
+        "fp64_plain": {"value": ps / t64, "ms_per_step": t64 * 1e3, "kernel_ms": k64,
+                       "price": r64["price"].tolist()},
+        "fp32_fused": {"kernel_ms": k32, "price": r32["price"].tolist(),
+                       "std_err": r32["std_err"].tolist()},
+        "speedup_fp32_vs_fp64": t64 / t32,
+    }
+    achieved = OPS_PER_STEP_HESTON32 * ps / (k32 * 1e-3) / 1e12
+    out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TOPS,
+                       "unit": "Tlane-op/s", "frac": achieved / PEAK_VALU_TOPS,
+                       "traffic": None,
+                       "kernel": "terminal_payoff_kernel<Heston32, CallPayoff>",
+                       "ops_per_path_step": OPS_PER_STEP_HESTON32,
+                       "kernel_ms": k32}
+
+    # ---- CPU baseline: the C oracle (OpenMP) on the same workload ----
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.build()
+        p = list(HESTON.values())
+        t0 = time.perf_counter()
+        term = O.terminals(1, p, T, STEPS, N_PATHS, SEED)
+        s1, _ = O.call_sums(term, STRIKES)
+        tc = time.perf_counter() - t0
+        ref = math.exp(-R * T) * s1 / N_PATHS
+        out["cpu_baseline"] = {"value": ps / tc, "unit": "path-steps/s",
+                               "cores": O.num_threads(), "kind": "port",
+                               "sample": f"full workload: {N_PATHS} Heston paths x {STEPS} "
+                                         "steps, FP64, calls at 90/100/110 (one run)",
+                               "seconds": tc}
+        out["rel_err_vs_fp64_oracle"] = {
+            "fp32": (np.abs(r32["price"] - ref) / ref).tolist(),
+            "fp64": (np.abs(r64["price"] - ref) / ref).tolist(),
+            "oracle_price": ref.tolist()}
+
+    if not args.no_extras:
+        def merton(prec):
+            return S.european_price("merton", MERTON, T, STEPS, N_PATHS, STRIKES, r=R,
+                                    seed=SEED, precision=prec)
+        m32 = time_calls(lambda: merton(32), 2, 1)
+        m64 = time_calls(lambda: merton(64), 2, 1)
+        out["merton"] = {"fp32_path_steps_per_s": ps / m32, "fp64_path_steps_per_s": ps / m64,
+                         "speedup": m64 / m32}
+        lsmc = {}
+        for n in (1_048_576, 4_194_304):
+            kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.20, t=1.0, steps=50, n_paths=n,
+                      is_put=True, poly_degree=3, seed=SEED)
+            tl = time_calls(lambda: S.lsmc_american_option_ex(**kw, precision=32), 2, 1)
+            price, se = S.lsmc_american_option_ex(**kw, precision=32)
+            tl64 = time_calls(lambda: S.lsmc_american_option(**kw), 2, 1)
+            p64, se64 = S.lsmc_american_option(**kw)
+            lsmc[str(n)] = {"fp32_ms": tl * 1e3, "fp32_price": price, "fp32_std_err": se,
+                            "fp64_ms": tl64 * 1e3, "fp64_price": p64, "fp64_std_err": se64}
+        out["lsmc_ms"] = lsmc
+
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

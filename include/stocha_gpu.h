@@ -1,0 +1,170 @@
+/*
+ * stocha_gpu.h -- C ABI of the MI355X (gfx950) Monte Carlo hot path.
+ *
+ * The entry points replace, one for one, the PyO3 functions of the
+ * reference's `stocha._stocha` module on the Monte Carlo path.  Each keeps
+ * the reference's argument meaning, defaults (applied by the binding),
+ * validation order and ValueError texts, and its output layout:
+ *
+ *   stocha_gbm                   <- src/lib.rs:205-252   (gbm)
+ *   stocha_heston                <- src/lib.rs:327-408   (heston, scheme="euler")
+ *   stocha_merton_jump_diffusion <- src/lib.rs:430-489   (merton_jump_diffusion)
+ *   stocha_multi_gbm             <- src/lib.rs:759-827   (multi_gbm)
+ *   stocha_lsmc_american_option  <- src/lib.rs:854-918   (lsmc_american_option)
+ *   stocha_greeks_pathwise       <- src/lib.rs:1177-1242 (greeks_pathwise)
+ *
+ * Conventions (all entry points):
+ *   - Plain C types only.  Integer arguments the reference takes as usize/u64
+ *     are uint64_t; negative Python ints are rejected by the binding before
+ *     the call (OverflowError, as PyO3 does).
+ *   - Return value: STOCHA_OK, or an error code with a NUL-terminated message
+ *     written to err[0..errlen) (the reference's ValueError text for
+ *     STOCHA_EINVAL).  Nothing is thrown across the ABI.
+ *   - Outputs are caller-allocated.  `where` says whether `out` is host
+ *     memory (STOCHA_HOST; the call returns after the copy-back) or device
+ *     memory on the current HIP device (STOCHA_DEVICE).  For the
+ *     path-returning functions out == NULL validates the arguments only
+ *     (no device work), so a binding can raise the reference's error before
+ *     it allocates the output.
+ *   - Work is issued on the stream set with stocha_set_stream (default: the
+ *     null stream) and every call synchronises that stream before it returns.
+ *   - Calls are serialised by an internal mutex; results do not depend on
+ *     launch configuration (block-split streams, fixed-order reductions).
+ */
+#ifndef STOCHA_GPU_H
+#define STOCHA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STOCHA_OK 0
+#define STOCHA_EINVAL 1  /* invalid argument -> ValueError */
+#define STOCHA_EDEVICE 2 /* HIP runtime error -> RuntimeError */
+#define STOCHA_ENOMEM 3  /* device allocation failed -> MemoryError */
+#define STOCHA_ENOTIMPL 4 /* outside this library's hot path -> NotImplementedError */
+
+#define STOCHA_HOST 0
+#define STOCHA_DEVICE 1
+
+/* Precision of the stepping arithmetic for the forms that offer a choice. */
+#define STOCHA_FP64 64
+#define STOCHA_FP32 32
+
+/* Model ids for the fused, batched and Greeks entry points. */
+#define STOCHA_MODEL_GBM 0
+#define STOCHA_MODEL_HESTON 1
+#define STOCHA_MODEL_MERTON 2
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Version string of this library ("stocha-gpu <semver>; reference 1.7.1"). */
+const char *stocha_version(void);
+
+/* Number of visible HIP devices (0 if none).  No reference counterpart. */
+int stocha_device_count(void);
+
+/* Issue all further work on `hip_stream` (a hipStream_t; NULL = null stream).
+ * No reference counterpart (the reference has no device). */
+int stocha_set_stream(void *hip_stream);
+
+/* Kernel-time probe: when enabled, each entry point records HIP events
+ * around its dominant kernel on the work stream; stocha_last_kernel_ms
+ * returns that kernel's duration for the last call (ms), and the number of
+ * launches it covered.  No reference counterpart (measurement only). */
+int stocha_timing_enable(int on);
+int stocha_last_kernel_ms(double *ms, uint64_t *launches);
+
+/* ---- drop-in path functions --------------------------------------------- */
+
+/* src/lib.rs:205-252 -> gbm.rs:42-107.  out: (n_paths, steps+1) f64,
+ * row-major; row i < base is path i, rows >= base are the antithetic
+ * mirrors of paths 0.. (base = (n_paths+1)/2 when antithetic). */
+int stocha_gbm(double s0, double mu, double sigma, double t, uint64_t steps, uint64_t n_paths,
+               uint64_t seed, int antithetic, double q, double *out, int where, char *err,
+               size_t errlen);
+
+/* src/lib.rs:327-408 -> heston.rs:32-87.  scheme: "euler" (full-truncation
+ * Euler).  "qe" passes validation as in the reference but is not part of this
+ * library's hot path: it returns STOCHA_ENOTIMPL with a message saying so.
+ * out: (n_paths, steps+1) f64 row-major. */
+int stocha_heston(double s0, double v0, double mu, double kappa, double theta, double xi,
+                  double rho, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
+                  const char *scheme, double *out, int where, char *err, size_t errlen);
+
+/* src/lib.rs:430-489 -> jump_diffusion.rs:48-105.  out: (n_paths, steps+1). */
+int stocha_merton_jump_diffusion(double s0, double mu, double sigma, double lambda_, double mu_j,
+                                 double sigma_j, double t, uint64_t steps, uint64_t n_paths,
+                                 uint64_t seed, double *out, int where, char *err,
+                                 size_t errlen);
+
+/* src/lib.rs:759-827 -> multi_gbm.rs:22-113 (+ copula/gaussian.rs:76-99).
+ * s0/mu/sigma have lengths n_s0/n_mu/n_sigma (checked like the reference);
+ * corr is (corr_rows, corr_cols) f64 row-major on the host.
+ * out: (n_paths, steps+1, d) f64 row-major. */
+int stocha_multi_gbm(const double *s0, uint64_t n_s0, const double *mu, uint64_t n_mu,
+                     const double *sigma, uint64_t n_sigma, const double *corr,
+                     uint64_t corr_rows, uint64_t corr_cols, double t, uint64_t steps,
+                     uint64_t n_paths, uint64_t seed, int antithetic, double *out, int where,
+                     char *err, size_t errlen);
+
+/* src/lib.rs:854-918 -> lsmc.rs:25-123.  Paths and the per-date solve in
+ * FP64 on the device; returns (price, std_err). */
+int stocha_lsmc_american_option(double s0, double k, double r, double sigma, double t,
+                                uint64_t steps, uint64_t n_paths, int is_put,
+                                uint64_t poly_degree, uint64_t seed, double *price,
+                                double *std_err, char *err, size_t errlen);
+
+/* src/lib.rs:1177-1242 -> greeks.rs:360-424.  greeks: n_greeks C strings
+ * ("delta"/"vega"; other names raise the reference's messages, lib.rs:920-938
+ * and :1213-1221).  values[i] receives the i-th requested greek. */
+int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double strike,
+                           int is_call, uint64_t n_paths, uint64_t n_steps,
+                           const char *const *greeks, uint64_t n_greeks, uint64_t seed,
+                           double *values, char *err, size_t errlen);
+
+/* ---- forms beside the drop-in (no reference counterpart; they extend the
+ *      same path and use the same streams) ------------------------------- */
+
+/* European calls priced in-register at n_strikes strikes (1..8) on the
+ * terminal prices of `model`, without storing paths.  params:
+ *   GBM    {s0, mu, sigma}                        (n_params 3)
+ *   Heston {s0, v0, mu, kappa, theta, xi, rho}    (n_params 7)
+ *   Merton {s0, mu, sigma, lambda, mu_j, sigma_j} (n_params 6)
+ * precision: STOCHA_FP64 (reference arithmetic, s *= exp(.) per step) or
+ * STOCHA_FP32 (log-space FP32 stepping, FP64 payoff sums).  Draws are the
+ * reference's block-split streams in both cases.  price[k] =
+ * exp(-r t) * mean payoff, std_err[k] = exp(-r t) * sqrt(var_{n-1} / n);
+ * sums (optional, 2*n_strikes) receives the raw FP64 sum and sum of squares of
+ * the undiscounted payoffs per strike. */
+int stocha_european_fused(int model, const double *params, uint64_t n_params, double t,
+                          uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
+                          const double *strikes, uint64_t n_strikes, double r, double *price,
+                          double *std_err, double *sums, char *err, size_t errlen);
+
+/* LSMC with a choice of path precision (FP32 paths: compensated FP64
+ * moment sums, FP64 solve).  Same validation as stocha_lsmc_american_option. */
+int stocha_lsmc_ex(double s0, double k, double r, double sigma, double t, uint64_t steps,
+                   uint64_t n_paths, int is_put, uint64_t poly_degree, uint64_t seed,
+                   int precision, double *price, double *std_err, char *err, size_t errlen);
+
+/* ---- random-number primitives (pcg64.rs:23-43, normal.rs:18-74) ---------
+ * Draws of the block-split streams of paths path0 .. path0+n_paths-1, where
+ * path i starts at Pcg64Dxsm::new(seed) advanced by i*block (the steppers'
+ * layout, e.g. gbm.rs:55-59).  out is (n_paths, n_draws) row-major, host.
+ * stocha_stream_normals: precision STOCHA_FP64 (the reference's values) or
+ * STOCHA_FP32 (same draws consumed, FP32 values widened to double);
+ * consumed (optional, n_paths) receives the u64 draws each path used. */
+int stocha_stream_u64(uint64_t seed, uint64_t block, uint64_t path0, uint64_t n_paths,
+                      uint64_t n_draws, uint64_t *out, char *err, size_t errlen);
+int stocha_stream_normals(uint64_t seed, uint64_t block, uint64_t path0, uint64_t n_paths,
+                          uint64_t n_draws, int precision, double *out, uint64_t *consumed,
+                          char *err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+"""ctypes loader for the CPU oracle (libstocha_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  The functions
+restate the reference's algorithms; see stocha_oracle.c for the file:line
+each follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libstocha_oracle.so")
+_lib = None
+
+_d = C.c_double
+_u = C.c_uint64
+_i = C.c_int
+_pd = C.POINTER(C.c_double)
+_pu = C.POINTER(C.c_uint64)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_stream_state.argtypes = [_u, _u, _u, _pu]
+        L.orc_advance.argtypes = [_pu, _u, _u]
+        L.orc_raw_u64.argtypes = [_u, _u, _u, _u, _pu]
+        L.orc_normals.argtypes = [_u, _u, _u, _u, _pd, _pu]
+        L.orc_gbm.argtypes = [_d, _d, _d, _d, _u, _u, _u, _i, _d, _pd]
+        L.orc_heston_euler.argtypes = [_d] * 8 + [_u, _u, _u, _pd]
+        L.orc_merton.argtypes = [_d] * 7 + [_u, _u, _u, _pd]
+        L.orc_multi_gbm.argtypes = [_u, _pd, _pd, _pd, _pd, _d, _u, _u, _u, _i, _pd,
+                                    C.c_char_p, C.c_size_t]
+        L.orc_cholesky.argtypes = [_u, _pd, _pd, C.c_char_p, C.c_size_t]
+        L.orc_terminals.argtypes = [_i, _pd, _d, _u, _u, _u, _pd]
+        L.orc_call_sums.argtypes = [_pd, _u, _pd, _i, _pd, _pd]
+        L.orc_call_sums.restype = None
+        L.orc_lsmc.argtypes = [_d, _d, _d, _d, _d, _u, _u, _i, _u, _u, _pd, _pd]
+        L.orc_lsmc_trace.argtypes = [_d, _d, _d, _d, _d, _u, _u, _i, _u, _u, _pd, _pu, _pd,
+                                     _pd]
+        L.orc_lstsq.argtypes = [_u, _u, _pd, _pd, _pd]
+        L.orc_greeks_pathwise.argtypes = [_d, _d, _d, _d, _d, _i, _u, _u, _u, _pd, _pd]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_pd):
+    return a.ctypes.data_as(t)
+
+
+# Block sizes per model (gbm.rs:55, heston.rs:48, jump_diffusion.rs:64, multi_gbm.rs:40)
+def block_gbm(steps):
+    return (steps + 1024) * 2
+
+
+def block_heston(steps):
+    return (steps + 1024) * 6
+
+
+def block_merton(steps):
+    return (steps + 1024) * 8
+
+
+def block_multi(steps, d):
+    return (steps * d + 1024) * 2
+
+
+def stream_state(seed, block, path):
+    out = np.zeros(4, np.uint64)
+    lib().orc_stream_state(seed, block, path, _p(out, _pu))
+    return out
+
+
+def raw_u64(seed, block, path, n):
+    out = np.zeros(n, np.uint64)
+    lib().orc_raw_u64(seed, block, path, n, _p(out, _pu))
+    return out
+
+
+def normals(seed, block, path, n):
+    out = np.zeros(n, np.float64)
+    used = np.zeros(1, np.uint64)
+    lib().orc_normals(seed, block, path, n, _p(out), _p(used, _pu))
+    return out, int(used[0])
+
+
+def gbm(s0, mu, sigma, t, steps, n_paths, seed=42, antithetic=False, q=0.0):
+    out = np.empty((n_paths, steps + 1), np.float64)
+    lib().orc_gbm(s0, mu, sigma, t, steps, n_paths, seed, int(antithetic), q, _p(out))
+    return out
+
+
+def heston(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed=42):
+    out = np.empty((n_paths, steps + 1), np.float64)
+    lib().orc_heston_euler(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed, _p(out))
+    return out
+
+
+def merton(s0, mu, sigma, lambda_, mu_j, sigma_j, t, steps, n_paths, seed=42):
+    out = np.empty((n_paths, steps + 1), np.float64)
+    lib().orc_merton(s0, mu, sigma, lambda_, mu_j, sigma_j, t, steps, n_paths, seed, _p(out))
+    return out
+
+
+def cholesky(a):
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    l = np.zeros_like(a)
+    err = C.create_string_buffer(256)
+    rc = lib().orc_cholesky(n, _p(a), _p(l), err, 256)
+    if rc:
+        raise ValueError(err.value.decode())
+    return l
+
+
+def multi_gbm(s0, mu, sigma, corr, t, steps, n_paths, seed=42, antithetic=False):
+    d = len(s0)
+    s0 = np.ascontiguousarray(s0, np.float64)
+    mu = np.ascontiguousarray(mu, np.float64)
+    sigma = np.ascontiguousarray(sigma, np.float64)
+    corr = np.ascontiguousarray(corr, np.float64)
+    out = np.empty((n_paths, steps + 1, d), np.float64)
+    err = C.create_string_buffer(256)
+    rc = lib().orc_multi_gbm(d, _p(s0), _p(mu), _p(sigma), _p(corr), t, steps, n_paths, seed,
+                             int(antithetic), _p(out), err, 256)
+    if rc:
+        raise ValueError(err.value.decode())
+    return out
+
+
+MODEL_GBM, MODEL_HESTON, MODEL_MERTON = 0, 1, 2
+
+
+def terminals(model, params, t, steps, n_paths, seed=42):
+    p = np.ascontiguousarray(params, np.float64)
+    out = np.empty(n_paths, np.float64)
+    rc = lib().orc_terminals(model, _p(p), t, steps, n_paths, seed, _p(out))
+    assert rc == 0
+    return out
+
+
+def call_sums(term, strikes):
+    term = np.ascontiguousarray(term, np.float64)
+    k = np.ascontiguousarray(strikes, np.float64)
+    s1 = np.zeros(len(k))
+    s2 = np.zeros(len(k))
+    lib().orc_call_sums(_p(term), len(term), _p(k), len(k), _p(s1), _p(s2))
+    return s1, s2
+
+
+def lsmc(s0, k, r, sigma, t, steps, n_paths, is_put=True, poly_degree=3, seed=42):
+    price = np.zeros(1)
+    se = np.zeros(1)
+    rc = lib().orc_lsmc(s0, k, r, sigma, t, steps, n_paths, int(is_put), poly_degree, seed,
+                        _p(price), _p(se))
+    assert rc == 0
+    return float(price[0]), float(se[0])
+
+
+def lsmc_trace(s0, k, r, sigma, t, steps, n_paths, is_put=True, poly_degree=3, seed=42):
+    coeffs = np.zeros((steps, poly_degree + 1))
+    itm = np.zeros(steps, np.uint64)
+    price = np.zeros(1)
+    se = np.zeros(1)
+    rc = lib().orc_lsmc_trace(s0, k, r, sigma, t, steps, n_paths, int(is_put), poly_degree, seed,
+                              _p(coeffs), _p(itm, _pu), _p(price), _p(se))
+    assert rc == 0
+    return coeffs, itm, float(price[0]), float(se[0])
+
+
+def lstsq(X, y):
+    X = np.ascontiguousarray(X, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    b = np.zeros(X.shape[1])
+    lib().orc_lstsq(X.shape[0], X.shape[1], _p(X), _p(y), _p(b))
+    return b
+
+
+def greeks_pathwise(s0, r, sigma, t, strike, is_call, n_paths, n_steps, seed=42):
+    d = np.zeros(1)
+    v = np.zeros(1)
+    lib().orc_greeks_pathwise(s0, r, sigma, t, strike, int(is_call), n_paths, n_steps, seed,
+                              _p(d), _p(v))
+    return float(d[0]), float(v[0])
+
+
+def num_threads():
+    return lib().orc_num_threads()

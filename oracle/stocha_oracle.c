@@ -1,0 +1,658 @@
+/*
+ * stocha_oracle.c -- CPU restatement of the reference's Monte Carlo hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; see stocha_oracle.h).  Compile with
+ * -O2 -ffp-contract=off so that every a*b+c is two roundings, as in Rust.
+ * Each function cites the reference file:line it follows.
+ */
+#include "stocha_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "zig_tables.h"
+
+typedef unsigned __int128 u128;
+
+static const double ZX[257] = {ZIG_X_VALUES};
+static const double ZY[257] = {ZIG_Y_VALUES};
+static const double ZR = ZIG_R_VALUE;
+
+/* ------------------------------------------------------------------------ */
+/* PCG64DXSM: rand_pcg 0.10.2 Lcg128CmDxsm64 as wrapped by pcg64.rs:23-43.    */
+/* ------------------------------------------------------------------------ */
+#define PCG_CHEAP_MUL 0xda942042e4dd58b5ull
+
+typedef struct {
+    u128 state;
+    u128 inc;
+} pcg_t;
+
+/* pcg64.rs:23-30: stream = seed + 0xda3e...4f5a; Lcg128CmDxsm64::new(state=seed, stream):
+ * inc = (stream << 1) | 1, state = seed + inc, then one LCG step. */
+static void pcg_new(pcg_t *g, uint64_t seed) {
+    u128 s = (u128)seed;
+    u128 stream = s + (((u128)0xda3e39cb94b95bdbull << 64) | 0x7ef55da5c0374f5aull);
+    g->inc = (stream << 1) | 1;
+    g->state = s + g->inc;
+    g->state = g->state * (u128)PCG_CHEAP_MUL + g->inc;
+}
+
+/* pcg64.rs:33-35 -> DXSM output of the pre-step state, then the LCG step. */
+static inline uint64_t pcg_next_u64(pcg_t *g) {
+    uint64_t hi = (uint64_t)(g->state >> 64);
+    uint64_t lo = (uint64_t)g->state | 1u;
+    hi ^= hi >> 32;
+    hi *= PCG_CHEAP_MUL;
+    hi ^= hi >> 48;
+    hi *= lo;
+    g->state = g->state * (u128)PCG_CHEAP_MUL + g->inc;
+    return hi;
+}
+
+/* pcg64.rs:37-39 */
+static inline double pcg_next_f64(pcg_t *g) {
+    return (double)(pcg_next_u64(g) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* pcg64.rs:41-43 -> rand_pcg advance: jump delta steps by square-and-multiply
+ * of the affine map s -> a*s + c (Brown, "Random number generation with
+ * arbitrary strides", 1994). */
+static void pcg_advance(pcg_t *g, u128 delta) {
+    u128 acc_mult = 1, acc_plus = 0;
+    u128 cur_mult = (u128)PCG_CHEAP_MUL, cur_plus = g->inc;
+    while (delta > 0) {
+        if (delta & 1) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+        delta >>= 1;
+    }
+    g->state = acc_mult * g->state + acc_plus;
+}
+
+static void pcg_for_path(pcg_t *g, uint64_t seed, uint64_t block, uint64_t path) {
+    pcg_new(g, seed);
+    pcg_advance(g, (u128)path * (u128)block);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ziggurat: normal.rs:18-67                                                 */
+/* ------------------------------------------------------------------------ */
+static double zig_tail(pcg_t *g) {
+    for (;;) {
+        double u1 = pcg_next_f64(g);
+        double u2 = pcg_next_f64(g);
+        if (u1 <= 0.0 || u2 <= 0.0) continue;
+        double x = -log(u1) / ZR;
+        if (-2.0 * log(u2) >= x * x) return x + ZR;
+    }
+}
+
+static inline double zig_normal(pcg_t *g) {
+    for (;;) {
+        uint64_t u = pcg_next_u64(g);
+        int layer = (int)(u & 0xFF);
+        double sign = (double)((u >> 8) & 1) * -2.0 + 1.0;
+        double u_frac = (double)(u >> 11) * (1.0 / 9007199254740992.0);
+        double x = u_frac * ZX[layer];
+        if (x < ZX[layer + 1]) return sign * x;
+        if (layer == 0) return sign * zig_tail(g);
+        double y = ZY[layer] + pcg_next_f64(g) * (ZY[layer + 1] - ZY[layer]);
+        if (y < exp(-0.5 * x * x)) return sign * x;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stream probes                                                             */
+/* ------------------------------------------------------------------------ */
+void orc_stream_state(uint64_t seed, uint64_t block, uint64_t path, uint64_t out4[4]) {
+    pcg_t g;
+    pcg_for_path(&g, seed, block, path);
+    out4[0] = (uint64_t)g.state;
+    out4[1] = (uint64_t)(g.state >> 64);
+    out4[2] = (uint64_t)g.inc;
+    out4[3] = (uint64_t)(g.inc >> 64);
+}
+
+void orc_advance(uint64_t st[4], uint64_t delta_lo, uint64_t delta_hi) {
+    pcg_t g;
+    g.state = ((u128)st[1] << 64) | st[0];
+    g.inc = ((u128)st[3] << 64) | st[2];
+    pcg_advance(&g, ((u128)delta_hi << 64) | delta_lo);
+    st[0] = (uint64_t)g.state;
+    st[1] = (uint64_t)(g.state >> 64);
+}
+
+void orc_raw_u64(uint64_t seed, uint64_t block, uint64_t path, uint64_t n, uint64_t *out) {
+    pcg_t g;
+    pcg_for_path(&g, seed, block, path);
+    for (uint64_t i = 0; i < n; i++) out[i] = pcg_next_u64(&g);
+}
+
+void orc_normals(uint64_t seed, uint64_t block, uint64_t path, uint64_t n, double *out,
+                 uint64_t *consumed) {
+    pcg_t g, g0;
+    pcg_for_path(&g, seed, block, path);
+    g0 = g;
+    for (uint64_t i = 0; i < n; i++) out[i] = zig_normal(&g);
+    if (consumed) {
+        /* count draws by stepping a copy until the states meet (n is small) */
+        uint64_t c = 0;
+        while (g0.state != g.state) {
+            g0.state = g0.state * (u128)PCG_CHEAP_MUL + g0.inc;
+            c++;
+        }
+        *consumed = c;
+    }
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* GBM: gbm.rs:42-107                                                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double drift, diffusion, s0;
+    uint64_t steps, block;
+} gbm_k;
+
+static void gbm_k_init(gbm_k *k, double s0, double mu, double sigma, double t, uint64_t steps,
+                       double q) {
+    double dt = t / (double)steps;
+    k->drift = (mu - q - 0.5 * sigma * sigma) * dt; /* gbm.rs:44 */
+    k->diffusion = sigma * sqrt(dt);                 /* gbm.rs:45 */
+    k->s0 = s0;
+    k->steps = steps;
+    k->block = (steps + 1024) * 2; /* gbm.rs:55 */
+}
+
+/* gbm.rs:57-75: one path; row (steps+1) or NULL (terminal only). */
+static double gbm_one(const gbm_k *k, uint64_t seed, uint64_t path, int negate, double *row) {
+    pcg_t g;
+    pcg_for_path(&g, seed, k->block, path);
+    double s = k->s0;
+    if (row) row[0] = s;
+    for (uint64_t j = 0; j < k->steps; j++) {
+        double z = negate ? -zig_normal(&g) : zig_normal(&g);
+        s *= exp(k->drift + k->diffusion * z);
+        if (row) row[j + 1] = s;
+    }
+    return s;
+}
+
+int orc_gbm(double s0, double mu, double sigma, double t, uint64_t steps, uint64_t n_paths,
+            uint64_t seed, int antithetic, double q, double *out) {
+    gbm_k k;
+    gbm_k_init(&k, s0, mu, sigma, t, steps, q);
+    uint64_t base = antithetic ? (n_paths + 1) / 2 : n_paths;
+    uint64_t w = steps + 1;
+    /* gbm.rs:83-97: rows [base paths..., antithetic mirrors...] truncated to n. */
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t r = 0; r < (int64_t)n_paths; r++) {
+        uint64_t row = (uint64_t)r;
+        if (row < base)
+            gbm_one(&k, seed, row, 0, out + row * w);
+        else
+            gbm_one(&k, seed, row - base, 1, out + row * w);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Heston full-truncation Euler: heston.rs:44-87                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double s0, v0, mu, kappa, theta, xi, rho, dt, sqrt_dt, rho_comp;
+    uint64_t steps, block;
+} heston_k;
+
+static void heston_k_init(heston_k *k, const double *p, double t, uint64_t steps) {
+    k->s0 = p[0];
+    k->v0 = p[1];
+    k->mu = p[2];
+    k->kappa = p[3];
+    k->theta = p[4];
+    k->xi = p[5];
+    k->rho = p[6];
+    k->dt = t / (double)steps;
+    k->sqrt_dt = sqrt(k->dt);
+    k->rho_comp = sqrt(1.0 - k->rho * k->rho);
+    k->steps = steps;
+    k->block = (steps + 1024) * 6; /* heston.rs:48 */
+}
+
+static double heston_one(const heston_k *k, uint64_t seed, uint64_t path, double *row) {
+    pcg_t g;
+    pcg_for_path(&g, seed, k->block, path);
+    double s = k->s0, v = k->v0;
+    if (row) row[0] = s;
+    for (uint64_t j = 0; j < k->steps; j++) { /* heston.rs:59-71 */
+        double dw1 = zig_normal(&g);
+        double dz = zig_normal(&g);
+        double dw2 = k->rho * dw1 + k->rho_comp * dz;
+        double v_plus = v > 0.0 ? v : 0.0;
+        double sqrt_v = sqrt(v_plus);
+        v += k->kappa * (k->theta - v_plus) * k->dt + k->xi * sqrt_v * k->sqrt_dt * dw2;
+        s *= exp((k->mu - 0.5 * v_plus) * k->dt + sqrt_v * k->sqrt_dt * dw1);
+        if (row) row[j + 1] = s;
+    }
+    return s;
+}
+
+int orc_heston_euler(double s0, double v0, double mu, double kappa, double theta, double xi,
+                     double rho, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
+                     double *out) {
+    double p[7] = {s0, v0, mu, kappa, theta, xi, rho};
+    heston_k k;
+    heston_k_init(&k, p, t, steps);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++)
+        heston_one(&k, seed, (uint64_t)i, out + (uint64_t)i * (steps + 1));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Merton jump-diffusion: jump_diffusion.rs:48-105                           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double s0, drift, diffusion, jump_prob, mu_j, sigma_j;
+    uint64_t steps, block;
+} merton_k;
+
+static void merton_k_init(merton_k *k, const double *p, double t, uint64_t steps) {
+    double s0 = p[0], mu = p[1], sigma = p[2], lambda = p[3], mu_j = p[4], sigma_j = p[5];
+    double dt = t / (double)steps;
+    double sqrt_dt = sqrt(dt);
+    double m_bar = exp(mu_j + 0.5 * sigma_j * sigma_j) - 1.0;    /* :53 */
+    k->drift = (mu - 0.5 * sigma * sigma - lambda * m_bar) * dt; /* :56 */
+    k->diffusion = sigma * sqrt_dt;
+    k->jump_prob = lambda * dt;
+    k->mu_j = mu_j;
+    k->sigma_j = sigma_j;
+    k->s0 = s0;
+    k->steps = steps;
+    k->block = (steps + 1024) * 8; /* :64 */
+}
+
+static double merton_one(const merton_k *k, uint64_t seed, uint64_t path, double *row) {
+    pcg_t g;
+    pcg_for_path(&g, seed, k->block, path);
+    double s = k->s0;
+    if (row) row[0] = s;
+    for (uint64_t j = 0; j < k->steps; j++) { /* :74-89 */
+        double z = zig_normal(&g);
+        double diffusion_return = k->drift + k->diffusion * z;
+        double u = pcg_next_f64(&g);
+        double jump_return = 0.0;
+        if (u < k->jump_prob) {
+            double z_j = zig_normal(&g);
+            jump_return = k->mu_j + k->sigma_j * z_j;
+        }
+        s *= exp(diffusion_return + jump_return);
+        if (row) row[j + 1] = s;
+    }
+    return s;
+}
+
+int orc_merton(double s0, double mu, double sigma, double lambda, double mu_j, double sigma_j,
+               double t, uint64_t steps, uint64_t n_paths, uint64_t seed, double *out) {
+    double p[6] = {s0, mu, sigma, lambda, mu_j, sigma_j};
+    merton_k k;
+    merton_k_init(&k, p, t, steps);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++)
+        merton_one(&k, seed, (uint64_t)i, out + (uint64_t)i * (steps + 1));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Terminal-only forms and payoff sums                                       */
+/* ------------------------------------------------------------------------ */
+int orc_terminals(int model, const double *p, double t, uint64_t steps, uint64_t n_paths,
+                  uint64_t seed, double *out) {
+    if (model == 0) {
+        gbm_k k;
+        gbm_k_init(&k, p[0], p[1], p[2], t, steps, p[3]);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t i = 0; i < (int64_t)n_paths; i++)
+            out[i] = gbm_one(&k, seed, (uint64_t)i, 0, NULL);
+    } else if (model == 1) {
+        heston_k k;
+        heston_k_init(&k, p, t, steps);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t i = 0; i < (int64_t)n_paths; i++)
+            out[i] = heston_one(&k, seed, (uint64_t)i, NULL);
+    } else if (model == 2) {
+        merton_k k;
+        merton_k_init(&k, p, t, steps);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t i = 0; i < (int64_t)n_paths; i++)
+            out[i] = merton_one(&k, seed, (uint64_t)i, NULL);
+    } else {
+        return 1;
+    }
+    return 0;
+}
+
+/* greeks.rs:12-23 (Payoff::eval for a call) summed in path order. */
+void orc_call_sums(const double *terminals, uint64_t n, const double *strikes, int n_strikes,
+                   double *sum, double *sumsq) {
+    for (int k = 0; k < n_strikes; k++) {
+        double s1 = 0.0, s2 = 0.0;
+        for (uint64_t i = 0; i < n; i++) {
+            double pay = terminals[i] - strikes[k];
+            pay = pay > 0.0 ? pay : 0.0;
+            s1 += pay;
+            s2 += pay * pay;
+        }
+        sum[k] = s1;
+        sumsq[k] = s2;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cholesky: copula/gaussian.rs:76-99 (row-wise, lower triangle only)        */
+/* ------------------------------------------------------------------------ */
+int orc_cholesky(uint64_t n, const double *a, double *l, char *err, size_t errlen) {
+    memset(l, 0, sizeof(double) * n * n);
+    for (uint64_t i = 0; i < n; i++) {
+        for (uint64_t j = 0; j <= i; j++) {
+            double sum = a[i * n + j];
+            for (uint64_t k = 0; k < j; k++) sum -= l[i * n + k] * l[j * n + k];
+            if (i == j) {
+                if (sum <= 0.0) {
+                    if (err)
+                        snprintf(err, errlen,
+                                 "matrix is not positive-definite (diagonal element %llu = %.17g)",
+                                 (unsigned long long)i, sum);
+                    return 1;
+                }
+                l[i * n + j] = sqrt(sum);
+            } else {
+                l[i * n + j] = sum / l[j * n + j];
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-asset GBM: multi_gbm.rs:22-113                                      */
+/* ------------------------------------------------------------------------ */
+int orc_multi_gbm(uint64_t d, const double *s0, const double *mu, const double *sigma,
+                  const double *corr, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
+                  int antithetic, double *out, char *err, size_t errlen) {
+    double *L = (double *)malloc(sizeof(double) * d * d);
+    double *drift = (double *)malloc(sizeof(double) * d);
+    double *diff = (double *)malloc(sizeof(double) * d);
+    if (!L || !drift || !diff) {
+        free(L);
+        free(drift);
+        free(diff);
+        return 2;
+    }
+    if (orc_cholesky(d, corr, L, err, errlen)) {
+        free(L);
+        free(drift);
+        free(diff);
+        return 1;
+    }
+    double dt = t / (double)steps;
+    double sqrt_dt = sqrt(dt);
+    for (uint64_t i = 0; i < d; i++) {
+        drift[i] = (mu[i] - 0.5 * sigma[i] * sigma[i]) * dt;
+        diff[i] = sigma[i] * sqrt_dt;
+    }
+    uint64_t base = antithetic ? (n_paths + 1) / 2 : n_paths;
+    uint64_t block = (steps * d + 1024) * 2; /* multi_gbm.rs:40 */
+    uint64_t row_len = (steps + 1) * d;
+    int rc = 0;
+#pragma omp parallel
+    {
+        double *z = (double *)malloc(sizeof(double) * d);
+        double *s = (double *)malloc(sizeof(double) * d);
+        if (!z || !s) {
+#pragma omp atomic write
+            rc = 2;
+        } else {
+#pragma omp for schedule(dynamic, 16)
+            for (int64_t r = 0; r < (int64_t)n_paths; r++) {
+                uint64_t row = (uint64_t)r;
+                int negate = row >= base;
+                uint64_t path = negate ? row - base : row;
+                pcg_t g;
+                pcg_for_path(&g, seed, block, path);
+                double *o = out + row * row_len;
+                for (uint64_t i = 0; i < d; i++) {
+                    s[i] = s0[i];
+                    o[i] = s0[i];
+                }
+                for (uint64_t j = 0; j < steps; j++) {
+                    for (uint64_t i = 0; i < d; i++) z[i] = zig_normal(&g);
+                    if (negate)
+                        for (uint64_t i = 0; i < d; i++) z[i] = -z[i];
+                    for (uint64_t i = 0; i < d; i++) {
+                        double eps = 0.0;
+                        for (uint64_t k = 0; k <= i; k++) eps += L[i * d + k] * z[k];
+                        s[i] *= exp(drift[i] + diff[i] * eps);
+                        o[(j + 1) * d + i] = s[i];
+                    }
+                }
+            }
+        }
+        free(z);
+        free(s);
+    }
+    free(L);
+    free(drift);
+    free(diff);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Least squares by Householder QR (the role of faer's qr().solve_lstsq at   */
+/* lsmc.rs:83-84).  X is m x n row-major, m >= n.                            */
+/* ------------------------------------------------------------------------ */
+int orc_lstsq(uint64_t m, uint64_t n, const double *X, const double *y, double *b) {
+    double *A = (double *)malloc(sizeof(double) * m * n);
+    double *r = (double *)malloc(sizeof(double) * m);
+    if (!A || !r) {
+        free(A);
+        free(r);
+        return 2;
+    }
+    memcpy(A, X, sizeof(double) * m * n);
+    memcpy(r, y, sizeof(double) * m);
+    for (uint64_t k = 0; k < n; k++) {
+        double norm = 0.0;
+        for (uint64_t i = k; i < m; i++) norm += A[i * n + k] * A[i * n + k];
+        norm = sqrt(norm);
+        if (norm == 0.0) continue;
+        double alpha = A[k * n + k] > 0 ? -norm : norm;
+        /* v = x - alpha e1, stored in column k below the diagonal */
+        double v0 = A[k * n + k] - alpha;
+        A[k * n + k] = v0;
+        double vnorm2 = v0 * v0;
+        for (uint64_t i = k + 1; i < m; i++) vnorm2 += A[i * n + k] * A[i * n + k];
+        if (vnorm2 == 0.0) {
+            A[k * n + k] = alpha;
+            continue;
+        }
+        for (uint64_t j = k + 1; j < n; j++) {
+            double dot = 0.0;
+            for (uint64_t i = k; i < m; i++) dot += A[i * n + k] * A[i * n + j];
+            double f = 2.0 * dot / vnorm2;
+            for (uint64_t i = k; i < m; i++) A[i * n + j] -= f * A[i * n + k];
+        }
+        double dot = 0.0;
+        for (uint64_t i = k; i < m; i++) dot += A[i * n + k] * r[i];
+        double f = 2.0 * dot / vnorm2;
+        for (uint64_t i = k; i < m; i++) r[i] -= f * A[i * n + k];
+        A[k * n + k] = alpha; /* R diagonal */
+    }
+    for (int64_t k = (int64_t)n - 1; k >= 0; k--) {
+        double acc = r[k];
+        for (uint64_t j = (uint64_t)k + 1; j < n; j++) acc -= A[k * n + j] * b[j];
+        b[k] = acc / A[k * n + k];
+    }
+    free(A);
+    free(r);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LSMC: lsmc.rs:25-123                                                      */
+/* ------------------------------------------------------------------------ */
+static double intrinsic(double s, double k, int is_put) { /* lsmc.rs:117-123 */
+    double v = is_put ? k - s : s - k;
+    return v > 0.0 ? v : 0.0;
+}
+
+/* compiler-rt __powidf2: the lowering of Rust's f64::powi (lsmc.rs:78,94). */
+static double powi_rt(double a, int b) {
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return r;
+}
+
+int orc_lsmc_trace(double s0, double k, double r, double sigma, double t, uint64_t steps,
+                   uint64_t n_paths, int is_put, uint64_t poly_degree, uint64_t seed,
+                   double *coeffs_out, uint64_t *itm_out, double *price, double *std_err) {
+    uint64_t n = n_paths, w = steps + 1, ncols = poly_degree + 1;
+    double dt = t / (double)steps;
+    double discount = exp(-r * dt);
+    double *paths = (double *)malloc(sizeof(double) * n * w);
+    double *cf = (double *)malloc(sizeof(double) * n);
+    uint64_t *itm = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    double *X = (double *)malloc(sizeof(double) * n * ncols);
+    double *y = (double *)malloc(sizeof(double) * n);
+    double coeffs[8];
+    if (!paths || !cf || !itm || !X || !y) {
+        free(paths);
+        free(cf);
+        free(itm);
+        free(X);
+        free(y);
+        return 2;
+    }
+    orc_gbm(s0, r, sigma, t, steps, n, seed, 0, 0.0, paths); /* lsmc.rs:29-39 */
+    for (uint64_t i = 0; i < n; i++) cf[i] = intrinsic(paths[i * w + steps], k, is_put);
+    if (coeffs_out)
+        for (uint64_t i = 0; i < steps * ncols; i++) coeffs_out[i] = NAN;
+    for (uint64_t step = steps - 1; step >= 1; step--) { /* lsmc.rs:50 */
+        for (uint64_t i = 0; i < n; i++) cf[i] *= discount;
+        uint64_t m = 0;
+        for (uint64_t i = 0; i < n; i++)
+            if (intrinsic(paths[i * w + step], k, is_put) > 0.0) itm[m++] = i;
+        if (itm_out) itm_out[step] = m;
+        if (m >= ncols + 1) {
+            double s_mean = 0.0;
+            for (uint64_t j = 0; j < m; j++) s_mean += paths[itm[j] * w + step];
+            s_mean = s_mean / (double)m;
+            double v = 0.0;
+            for (uint64_t j = 0; j < m; j++) {
+                double dlt = paths[itm[j] * w + step] - s_mean;
+                v += powi_rt(dlt, 2);
+            }
+            v = v / (double)m;
+            double s_std = sqrt(v);
+            if (s_std < 1e-12) s_std = 1e-12;
+            for (uint64_t j = 0; j < m; j++) {
+                double x = (paths[itm[j] * w + step] - s_mean) / s_std;
+                for (uint64_t c = 0; c < ncols; c++) X[j * ncols + c] = powi_rt(x, (int)c);
+                y[j] = cf[itm[j]];
+            }
+            orc_lstsq(m, ncols, X, y, coeffs);
+            if (coeffs_out)
+                for (uint64_t c = 0; c < ncols; c++) coeffs_out[step * ncols + c] = coeffs[c];
+            for (uint64_t j = 0; j < m; j++) { /* lsmc.rs:91-98 */
+                double sp = paths[itm[j] * w + step];
+                double x = (sp - s_mean) / s_std;
+                double cont = 0.0;
+                for (uint64_t c = 0; c < ncols; c++) cont += coeffs[c] * powi_rt(x, (int)c);
+                double ex = intrinsic(sp, k, is_put);
+                if (ex > (cont > 0.0 ? cont : 0.0)) cf[itm[j]] = ex;
+            }
+        }
+        if (step == 1) break;
+    }
+    for (uint64_t i = 0; i < n; i++) cf[i] *= discount;
+    double mean = 0.0;
+    for (uint64_t i = 0; i < n; i++) mean += cf[i];
+    mean = mean / (double)n;
+    double se = 0.0;
+    if (n > 1) {
+        double var = 0.0;
+        for (uint64_t i = 0; i < n; i++) var += powi_rt(cf[i] - mean, 2);
+        var = var / (double)(n - 1);
+        se = sqrt(var / (double)n);
+    }
+    *price = mean;
+    *std_err = se;
+    free(paths);
+    free(cf);
+    free(itm);
+    free(X);
+    free(y);
+    return 0;
+}
+
+int orc_lsmc(double s0, double k, double r, double sigma, double t, uint64_t steps,
+             uint64_t n_paths, int is_put, uint64_t poly_degree, uint64_t seed, double *price,
+             double *std_err) {
+    return orc_lsmc_trace(s0, k, r, sigma, t, steps, n_paths, is_put, poly_degree, seed, NULL,
+                          NULL, price, std_err);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pathwise Greeks: greeks.rs:360-424 (GBM with mu = r, terminal prices)     */
+/* ------------------------------------------------------------------------ */
+int orc_greeks_pathwise(double s0, double r, double sigma, double t, double strike, int is_call,
+                        uint64_t n_paths, uint64_t steps, uint64_t seed, double *delta,
+                        double *vega) {
+    double *term = (double *)malloc(sizeof(double) * n_paths);
+    if (!term) return 2;
+    double p[4] = {s0, r, sigma, 0.0};
+    orc_terminals(0, p, t, steps, n_paths, seed, term);
+    double discount = exp(-r * t);
+    double n = (double)n_paths;
+    double sd = 0.0, sv = 0.0;
+    for (uint64_t i = 0; i < n_paths; i++) {
+        double s_t = term[i];
+        int in_money = is_call ? s_t > strike : s_t < strike;
+        double sign = is_call ? 1.0 : -1.0;
+        if (in_money) sd += sign * s_t / s0;
+    }
+    for (uint64_t i = 0; i < n_paths; i++) {
+        double s_t = term[i];
+        int in_money = is_call ? s_t > strike : s_t < strike;
+        double sign = is_call ? 1.0 : -1.0;
+        if (in_money) {
+            double log_return = log(s_t / s0);
+            sv += sign * s_t * (log_return - (r + 0.5 * sigma * sigma) * t) / sigma;
+        }
+    }
+    *delta = discount * sd / n;
+    *vega = discount * sv / n;
+    free(term);
+    return 0;
+}

@@ -1,0 +1,213 @@
+"""stocha_amd -- the Monte Carlo hot path of stocha, on MI355X (gfx950).
+
+A drop-in for these functions of the reference's Python API
+(python/stocha/__init__.py), same signatures, defaults, return types and
+errors:
+
+    gbm                    (__init__.py:180-231)
+    heston                 (__init__.py:276-324; scheme "euler")
+    merton_jump_diffusion  (__init__.py:327-373)
+    multi_gbm              (__init__.py:574-627)
+    lsmc_american_option   (__init__.py:630-671)
+    greeks_pathwise        (__init__.py:725-769)
+
+They run on the GPU through the C ABI of libstocha_gpu.so (hand-written HIP
+kernels); there is no CPU fallback.  Forms beside the drop-in API:
+
+    european_price         in-register European calls, FP64 or FP32 stepping
+    lsmc_american_option_ex  LSMC with FP32 paths
+
+The reference's other functions (RNG, sobol, halton, copulas, Hull-White,
+SABR, Heston COS / calibration, SSVI, exotics, var_cvar, greeks_fd) are not
+part of this package: calling them raises NotImplementedError.
+"""
+from __future__ import annotations
+
+from typing import Literal, Sequence
+
+import ctypes as _C
+
+import numpy as np
+
+from . import _lib
+from . import _stocha as _S
+from ._stocha import __version__
+
+__all__ = [
+    "gbm",
+    "heston",
+    "merton_jump_diffusion",
+    "multi_gbm",
+    "lsmc_american_option",
+    "greeks_pathwise",
+    "european_price",
+    "lsmc_american_option_ex",
+    "__version__",
+]
+
+
+def gbm(s0: float, mu: float, sigma: float, t: float, steps: int, n_paths: int,
+        seed: int = 42, antithetic: bool = False, q: float = 0.0) -> np.ndarray:
+    """Simulate GBM paths (reference: python/stocha/__init__.py:180-231).
+
+    Returns a (n_paths, steps + 1) float64 array; column 0 is s0.  Path i
+    uses the reference's block-split PCG64DXSM stream i, so the values equal
+    the reference's for the same seed.
+    """
+    return _S.gbm(s0=s0, mu=mu, sigma=sigma, t=t, steps=steps, n_paths=n_paths, seed=seed,
+                  antithetic=antithetic, q=q)
+
+
+def heston(s0: float, v0: float, mu: float, kappa: float, theta: float, xi: float, rho: float,
+           t: float, steps: int, n_paths: int, seed: int = 42,
+           scheme: Literal["euler", "qe"] = "euler") -> np.ndarray:
+    """Simulate Heston paths, full-truncation Euler (reference:
+    python/stocha/__init__.py:276-324).  scheme="qe" raises NotImplementedError."""
+    return _S.heston(s0=s0, v0=v0, mu=mu, kappa=kappa, theta=theta, xi=xi, rho=rho, t=t,
+                     steps=steps, n_paths=n_paths, seed=seed, scheme=scheme)
+
+
+def merton_jump_diffusion(s0: float, mu: float, sigma: float, lambda_: float, mu_j: float,
+                          sigma_j: float, t: float, steps: int, n_paths: int,
+                          seed: int = 42) -> np.ndarray:
+    """Simulate Merton jump-diffusion paths (reference: python/stocha/__init__.py:327-373)."""
+    return _S.merton_jump_diffusion(s0=s0, mu=mu, sigma=sigma, lambda_=lambda_, mu_j=mu_j,
+                                    sigma_j=sigma_j, t=t, steps=steps, n_paths=n_paths,
+                                    seed=seed)
+
+
+def multi_gbm(s0: list[float], mu: list[float], sigma: list[float], corr: np.ndarray, t: float,
+              steps: int, n_paths: int, seed: int = 42, antithetic: bool = False) -> np.ndarray:
+    """Simulate correlated multi-asset GBM (reference: python/stocha/__init__.py:574-627).
+
+    Returns (n_paths, steps + 1, n_assets) float64.
+    """
+    return _S.multi_gbm(s0=list(s0), mu=list(mu), sigma=list(sigma),
+                        corr=np.asarray(corr, dtype=np.float64), t=t, steps=steps,
+                        n_paths=n_paths, seed=seed, antithetic=antithetic)
+
+
+def lsmc_american_option(s0: float, k: float, r: float, sigma: float, t: float, steps: int,
+                         n_paths: int, is_put: bool = True, poly_degree: int = 3,
+                         seed: int = 42) -> tuple[float, float]:
+    """Longstaff-Schwartz American option, FP64 paths and solve (reference:
+    python/stocha/__init__.py:630-671).  Returns (price, std_err)."""
+    return _S.lsmc_american_option(s0=s0, k=k, r=r, sigma=sigma, t=t, steps=steps,
+                                   n_paths=n_paths, is_put=is_put, poly_degree=poly_degree,
+                                   seed=seed)
+
+
+def greeks_pathwise(s0: float, r: float, sigma: float, t: float, strike: float, is_call: bool,
+                    n_paths: int, n_steps: int, greeks: list[str],
+                    seed: int = 42) -> dict[str, float]:
+    """Pathwise (IPA) delta and vega for GBM (reference: python/stocha/__init__.py:725-769)."""
+    return _S.greeks_pathwise(s0=s0, r=r, sigma=sigma, t=t, strike=strike, is_call=is_call,
+                              n_paths=n_paths, n_steps=n_steps, greeks=greeks, seed=seed)
+
+
+# ------------------------------------------------------------------ extras --
+_MODELS = {"gbm": (_lib.MODEL_GBM, ("s0", "mu", "sigma")),
+           "heston": (_lib.MODEL_HESTON, ("s0", "v0", "mu", "kappa", "theta", "xi", "rho")),
+           "merton": (_lib.MODEL_MERTON, ("s0", "mu", "sigma", "lambda_", "mu_j", "sigma_j"))}
+
+
+def european_price(model: str, params: dict, t: float, steps: int, n_paths: int,
+                   strikes: Sequence[float], r: float, seed: int = 42,
+                   precision: int = 64) -> dict:
+    """European calls at up to 8 strikes, priced in-register on the GPU.
+
+    model: "gbm" | "heston" | "merton"; params holds the model's parameters
+    under the names of the corresponding path function.  precision 64 steps
+    with the reference's FP64 arithmetic; 32 steps the log-price in FP32 with
+    FP64 payoff sums.  Both use the reference's block-split streams, so they
+    price the same paths.  Returns {"price": array, "std_err": array,
+    "sum": array, "sumsq": array} (one entry per strike).
+    """
+    if model not in _MODELS:
+        raise ValueError(f"Unknown model: '{model}'. Supported: gbm, heston, merton")
+    mid, names = _MODELS[model]
+    missing = [n for n in names if n not in params]
+    if missing:
+        raise ValueError(f"missing parameter(s) for {model}: {', '.join(missing)}")
+    p = np.array([_S._f64(params[n], n) for n in names], dtype=np.float64)
+    k = np.ascontiguousarray([_S._f64(v, "strikes") for v in strikes], dtype=np.float64)
+    steps, n_paths, seed = _S._uint(steps, "steps"), _S._uint(n_paths, "n_paths"), \
+        _S._uint(seed, "seed")
+    lib, eb = _lib.load(), _lib.ErrBuf()
+    ns = len(k)
+    price = np.zeros(max(ns, 1))
+    se = np.zeros(max(ns, 1))
+    sums = np.zeros(2 * max(ns, 1))
+    rc = lib.stocha_european_fused(mid, _S._pd(p), len(p), _S._f64(t, "t"), steps, n_paths,
+                                   seed, int(precision), _S._pd(k), ns, _S._f64(r, "r"),
+                                   _S._pd(price), _S._pd(se), _S._pd(sums), eb.buf, eb.n)
+    _lib.check(rc, eb)
+    return {"price": price[:ns], "std_err": se[:ns], "sum": sums[0:2 * ns:2],
+            "sumsq": sums[1:2 * ns:2]}
+
+
+def lsmc_american_option_ex(s0: float, k: float, r: float, sigma: float, t: float, steps: int,
+                            n_paths: int, is_put: bool = True, poly_degree: int = 3,
+                            seed: int = 42, precision: int = 32) -> tuple[float, float]:
+    """lsmc_american_option with a choice of path precision (32: FP32 paths,
+    FP64 moment sums and solve)."""
+    return _S._lsmc(s0, k, r, sigma, t, steps, n_paths, is_put, poly_degree, seed,
+                    int(precision))
+
+
+def last_kernel_ms() -> tuple[float, int]:
+    """Duration of the dominant kernel(s) of the last call, when timing is on."""
+    lib = _lib.load()
+    ms, n = _C.c_double(), _C.c_uint64()
+    lib.stocha_last_kernel_ms(_C.byref(ms), _C.byref(n))
+    return ms.value, n.value
+
+
+def enable_kernel_timing(on: bool = True) -> None:
+    if _lib.load().stocha_timing_enable(int(on)) != 0:
+        raise RuntimeError("could not create HIP events")
+
+
+def device_count() -> int:
+    return int(_lib.load().stocha_device_count())
+
+
+def _not_here(name):
+    def f(*_a, **_k):
+        raise NotImplementedError(
+            f"stocha.{name} is not part of the GPU Monte Carlo hot path implemented by "
+            "stocha_amd")
+    f.__name__ = name
+    return f
+
+
+for _name in ("RNG", "sobol", "halton", "var_cvar", "gaussian_copula", "student_t_copula",
+              "hull_white", "sabr_implied_vol", "sabr_calibrate", "greeks_fd", "heston_price",
+              "heston_calibrate", "ssvi_calibrate", "ssvi_implied_vol", "ssvi_local_vol",
+              "barrier_price", "asian_price", "lookback_price"):
+    globals()[_name] = _not_here(_name)
+del _name
+
+
+def stream_u64(seed: int, block: int, path0: int, n_paths: int, n_draws: int) -> np.ndarray:
+    """Raw PCG64DXSM draws of block-split streams path0.. (device-generated)."""
+    out = np.zeros((n_paths, n_draws), dtype=np.uint64)
+    lib, eb = _lib.load(), _lib.ErrBuf()
+    rc = lib.stocha_stream_u64(seed, block, path0, n_paths, n_draws,
+                               out.ctypes.data_as(_C.POINTER(_C.c_uint64)), eb.buf, eb.n)
+    _lib.check(rc, eb)
+    return out
+
+
+def stream_normals(seed: int, block: int, path0: int, n_paths: int, n_draws: int,
+                   precision: int = 64) -> tuple[np.ndarray, np.ndarray]:
+    """Ziggurat normals of block-split streams (device-generated) and the
+    number of u64 draws each path consumed."""
+    out = np.zeros((n_paths, n_draws), dtype=np.float64)
+    used = np.zeros(n_paths, dtype=np.uint64)
+    lib, eb = _lib.load(), _lib.ErrBuf()
+    rc = lib.stocha_stream_normals(seed, block, path0, n_paths, n_draws, int(precision),
+                                   out.ctypes.data_as(_C.POINTER(_C.c_double)),
+                                   used.ctypes.data_as(_C.POINTER(_C.c_uint64)), eb.buf, eb.n)
+    _lib.check(rc, eb)
+    return out, used

@@ -1,0 +1,186 @@
+"""Binding layer: the hot-path functions of the reference's `stocha._stocha`.
+
+Mirrors the PyO3 functions of src/lib.rs (signatures, keyword names,
+defaults, argument extraction and error types) and calls the C ABI of
+libstocha_gpu.so.  Argument extraction follows PyO3's rules: floats accept
+anything with __float__ (PyFloat_AsDouble), usize/u64 arguments take Python
+ints only (TypeError otherwise) and raise OverflowError when negative or
+too large, bool arguments take bool only, `scheme` takes str, `greeks` takes a
+list.  The range checks and their ValueError texts live in the C library and
+run before any device work.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import operator
+
+import numpy as np
+
+from . import _lib as L
+
+__version__ = "1.7.1"  # the reference version whose API this module mirrors
+
+
+# ------------------------------------------------------------ extraction --
+def _f64(x, name):
+    if isinstance(x, (bool, np.bool_)):
+        return float(x)
+    try:
+        return float(x) if isinstance(x, (int, float, np.floating, np.integer)) else float(
+            x.__float__())
+    except AttributeError:
+        raise TypeError(f"argument '{name}': must be real number, not {type(x).__name__}")
+
+
+def _uint(x, name, bits=64):
+    if isinstance(x, (float, np.floating)) or not hasattr(x, "__index__"):
+        raise TypeError(f"argument '{name}': '{type(x).__name__}' object cannot be interpreted "
+                        "as an integer")
+    v = operator.index(x)
+    if v < 0:
+        raise OverflowError(f"argument '{name}': can't convert negative int to unsigned")
+    if v >= 1 << bits:
+        raise OverflowError(f"argument '{name}': int too big to convert")
+    return v
+
+
+def _bool(x, name):
+    if isinstance(x, (bool, np.bool_)):
+        return bool(x)
+    raise TypeError(f"argument '{name}': '{type(x).__name__}' object is not an instance of "
+                    "'bool'")
+
+
+def _vec_f64(x, name):
+    if isinstance(x, (str, bytes)):
+        raise TypeError(f"argument '{name}': Can't extract `str` to `Vec`")
+    return np.ascontiguousarray([_f64(v, name) for v in x], dtype=np.float64)
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ----------------------------------------------------------- entry points --
+def gbm(s0, mu, sigma, t, steps, n_paths, seed=42, antithetic=False, q=0.0):
+    """src/lib.rs:205-252 (signature :206).  Returns (n_paths, steps+1) float64."""
+    s0, mu, sigma, t = (_f64(s0, "s0"), _f64(mu, "mu"), _f64(sigma, "sigma"), _f64(t, "t"))
+    steps, n_paths = _uint(steps, "steps"), _uint(n_paths, "n_paths")
+    seed, antithetic, q = _uint(seed, "seed"), _bool(antithetic, "antithetic"), _f64(q, "q")
+    lib, eb = L.load(), L.ErrBuf()
+    # out=NULL validates only, so a bad argument never allocates the output
+    L.check(lib.stocha_gbm(s0, mu, sigma, t, steps, n_paths, seed, int(antithetic), q, None,
+                           L.STOCHA_HOST, eb.buf, eb.n), eb)
+    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    rc = lib.stocha_gbm(s0, mu, sigma, t, steps, n_paths, seed, int(antithetic), q,
+                        out.ctypes.data, L.STOCHA_HOST, eb.buf, eb.n)
+    L.check(rc, eb)
+    return out
+
+
+def heston(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed=42, scheme="euler"):
+    """src/lib.rs:327-408 (signature :328).  Returns (n_paths, steps+1) float64."""
+    vals = [_f64(v, n) for v, n in ((s0, "s0"), (v0, "v0"), (mu, "mu"), (kappa, "kappa"),
+                                     (theta, "theta"), (xi, "xi"), (rho, "rho"), (t, "t"))]
+    steps, n_paths, seed = _uint(steps, "steps"), _uint(n_paths, "n_paths"), _uint(seed, "seed")
+    if not isinstance(scheme, str):
+        raise TypeError(f"argument 'scheme': '{type(scheme).__name__}' object cannot be "
+                        "converted to 'PyString'")
+    lib, eb = L.load(), L.ErrBuf()
+    sch = scheme.encode()
+    L.check(lib.stocha_heston(*vals, steps, n_paths, seed, sch, None, L.STOCHA_HOST, eb.buf,
+                              eb.n), eb)
+    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    rc = lib.stocha_heston(*vals, steps, n_paths, seed, sch, out.ctypes.data, L.STOCHA_HOST,
+                           eb.buf, eb.n)
+    L.check(rc, eb)
+    return out
+
+
+def merton_jump_diffusion(s0, mu, sigma, lambda_, mu_j, sigma_j, t, steps, n_paths, seed=42):
+    """src/lib.rs:430-489 (signature :431).  Returns (n_paths, steps+1) float64."""
+    vals = [_f64(v, n) for v, n in ((s0, "s0"), (mu, "mu"), (sigma, "sigma"),
+                                     (lambda_, "lambda_"), (mu_j, "mu_j"),
+                                     (sigma_j, "sigma_j"), (t, "t"))]
+    steps, n_paths, seed = _uint(steps, "steps"), _uint(n_paths, "n_paths"), _uint(seed, "seed")
+    lib, eb = L.load(), L.ErrBuf()
+    L.check(lib.stocha_merton_jump_diffusion(*vals, steps, n_paths, seed, None, L.STOCHA_HOST,
+                                             eb.buf, eb.n), eb)
+    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    rc = lib.stocha_merton_jump_diffusion(*vals, steps, n_paths, seed, out.ctypes.data,
+                                          L.STOCHA_HOST, eb.buf, eb.n)
+    L.check(rc, eb)
+    return out
+
+
+def multi_gbm(s0, mu, sigma, corr, t, steps, n_paths, seed=42, antithetic=False):
+    """src/lib.rs:759-827 (signature :760).  Returns (n_paths, steps+1, d) float64."""
+    s0, mu, sigma = _vec_f64(s0, "s0"), _vec_f64(mu, "mu"), _vec_f64(sigma, "sigma")
+    if not isinstance(corr, np.ndarray) or corr.dtype != np.float64:
+        raise TypeError("argument 'corr': expected a numpy.ndarray of float64")
+    if corr.ndim != 2:
+        raise TypeError(f"argument 'corr': expected a 2-dimensional array, got {corr.ndim}")
+    corr_c = np.ascontiguousarray(corr)
+    t = _f64(t, "t")
+    steps, n_paths = _uint(steps, "steps"), _uint(n_paths, "n_paths")
+    seed, antithetic = _uint(seed, "seed"), _bool(antithetic, "antithetic")
+    lib, eb = L.load(), L.ErrBuf()
+    d = len(s0)
+
+    def call(ptr):
+        return lib.stocha_multi_gbm(_pd(s0), len(s0), _pd(mu), len(mu), _pd(sigma), len(sigma),
+                                    _pd(corr_c), corr_c.shape[0], corr_c.shape[1], t, steps,
+                                    n_paths, seed, int(antithetic), ptr, L.STOCHA_HOST, eb.buf,
+                                    eb.n)
+
+    L.check(call(None), eb)
+    out = np.empty((n_paths, steps + 1, d), dtype=np.float64)
+    L.check(call(out.ctypes.data), eb)
+    return out
+
+
+def lsmc_american_option(s0, k, r, sigma, t, steps, n_paths, is_put=True, poly_degree=3,
+                         seed=42):
+    """src/lib.rs:854-918 (signature :855).  Returns (price, std_err)."""
+    return _lsmc(s0, k, r, sigma, t, steps, n_paths, is_put, poly_degree, seed, L.STOCHA_FP64)
+
+
+def _lsmc(s0, k, r, sigma, t, steps, n_paths, is_put, poly_degree, seed, precision):
+    vals = [_f64(v, n) for v, n in ((s0, "s0"), (k, "k"), (r, "r"), (sigma, "sigma"),
+                                     (t, "t"))]
+    steps, n_paths = _uint(steps, "steps"), _uint(n_paths, "n_paths")
+    is_put = _bool(is_put, "is_put")
+    poly_degree, seed = _uint(poly_degree, "poly_degree"), _uint(seed, "seed")
+    lib, eb = L.load(), L.ErrBuf()
+    price, se = C.c_double(), C.c_double()
+    rc = lib.stocha_lsmc_ex(*vals, steps, n_paths, int(is_put), poly_degree, seed, precision,
+                            C.byref(price), C.byref(se), eb.buf, eb.n)
+    L.check(rc, eb)
+    return float(price.value), float(se.value)
+
+
+def greeks_pathwise(s0, r, sigma, t, strike, is_call, n_paths, n_steps, greeks, seed=42):
+    """src/lib.rs:1177-1242 (signature :1178).  Returns {name: value} in request order."""
+    vals = [_f64(v, n) for v, n in ((s0, "s0"), (r, "r"), (sigma, "sigma"), (t, "t"),
+                                     (strike, "strike"))]
+    is_call = _bool(is_call, "is_call")
+    n_paths, n_steps = _uint(n_paths, "n_paths"), _uint(n_steps, "n_steps")
+    if not isinstance(greeks, list):
+        raise TypeError(f"argument 'greeks': '{type(greeks).__name__}' object cannot be "
+                        "converted to 'PyList'")
+    names = []
+    for g in greeks:
+        if not isinstance(g, str):
+            raise TypeError(f"'{type(g).__name__}' object cannot be converted to 'PyString'")
+        names.append(g)
+    seed = _uint(seed, "seed")
+    lib, eb = L.load(), L.ErrBuf()
+    arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+    out = np.zeros(max(1, len(names)), dtype=np.float64)
+    rc = lib.stocha_greeks_pathwise(*vals, int(is_call), n_paths, n_steps, arr, len(names), seed,
+                                    _pd(out), eb.buf, eb.n)
+    L.check(rc, eb)
+    res = {}
+    for i, n in enumerate(names):
+        res[n] = float(out[i])
+    return res

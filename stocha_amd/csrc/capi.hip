@@ -1,0 +1,426 @@
+// capi.hip -- the extern "C" entry points declared in include/stocha_gpu.h.
+//
+// Each entry point validates its arguments in the order and with the
+// messages of the reference's PyO3 function (src/lib.rs, cited per
+// function), computes the per-launch constants with the reference's
+// expressions (gbm.rs:43-45, heston.rs:45-48, jump_diffusion.rs:49-64,
+// multi_gbm.rs:24-40, lsmc.rs:26-27), sets up the block-split stream table
+// and launches the kernels on the work stream.
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "multi.hpp"
+
+namespace stocha {
+
+Globals& globals() {
+  static Globals g;
+  return g;
+}
+
+GbmConst gbm_const(double s0, double mu, double sigma, double t, uint64_t steps, double q) {
+  GbmConst c;
+  const double dt = t / (double)steps;
+  c.s0 = s0;
+  c.drift = (mu - q - 0.5 * sigma * sigma) * dt;  // gbm.rs:44
+  c.diffusion = sigma * sqrt(dt);                  // gbm.rs:45
+  c.drift_f = (float)c.drift;
+  c.diffusion_f = (float)c.diffusion;
+  return c;
+}
+
+HestonConst heston_const(const double* p, double t, uint64_t steps) {
+  HestonConst c;
+  c.s0 = p[0];
+  c.v0 = p[1];
+  c.mu = p[2];
+  c.kappa = p[3];
+  c.theta = p[4];
+  c.xi = p[5];
+  c.rho = p[6];
+  c.dt = t / (double)steps;              // heston.rs:45
+  c.sqrt_dt = sqrt(c.dt);                // :46
+  c.rho_comp = sqrt(1.0 - c.rho * c.rho);  // :47
+  c.v0_f = (float)c.v0;
+  c.rho_f = (float)c.rho;
+  c.rho_comp_f = (float)c.rho_comp;
+  c.kappa_dt_f = (float)(c.kappa * c.dt);
+  c.theta_f = (float)c.theta;
+  c.xi_sqdt_f = (float)(c.xi * c.sqrt_dt);
+  c.mu_dt_f = (float)(c.mu * c.dt);
+  c.half_dt_f = (float)(0.5 * c.dt);
+  c.sqrt_dt_f = (float)c.sqrt_dt;
+  return c;
+}
+
+MertonConst merton_const(const double* p, double t, uint64_t steps) {
+  MertonConst c;
+  const double s0 = p[0], mu = p[1], sigma = p[2], lambda = p[3], mu_j = p[4], sigma_j = p[5];
+  const double dt = t / (double)steps;                            // jump_diffusion.rs:49
+  const double sqrt_dt = sqrt(dt);                                // :50
+  const double m_bar = exp(mu_j + 0.5 * sigma_j * sigma_j) - 1.0;  // :53
+  c.s0 = s0;
+  c.drift = (mu - 0.5 * sigma * sigma - lambda * m_bar) * dt;  // :56
+  c.diffusion = sigma * sqrt_dt;                               // :57
+  c.jump_prob = lambda * dt;                                   // :60
+  c.mu_j = mu_j;
+  c.sigma_j = sigma_j;
+  c.drift_f = (float)c.drift;
+  c.diffusion_f = (float)c.diffusion;
+  c.mu_j_f = (float)mu_j;
+  c.sigma_j_f = (float)sigma_j;
+  return c;
+}
+
+// Rust's `{}` Display for f64: shortest round-trip digits, never an
+// exponent (used in the Cholesky error text, copula/gaussian.rs:87-90).
+std::string rust_f64(double v) {
+  if (isnan(v)) return "NaN";
+  if (isinf(v)) return v > 0 ? "inf" : "-inf";
+  char buf[64];
+  int prec = 1;
+  for (; prec <= 17; ++prec) {
+    snprintf(buf, sizeof buf, "%.*e", prec - 1, v);
+    if (strtod(buf, nullptr) == v) break;
+  }
+  // buf = [-]d.ddddde[+-]XX -> digits and decimal exponent
+  std::string s(buf);
+  bool neg = s[0] == '-';
+  if (neg) s = s.substr(1);
+  size_t epos = s.find('e');
+  int e10 = atoi(s.c_str() + epos + 1);
+  std::string digits;
+  for (size_t i = 0; i < epos; ++i)
+    if (s[i] != '.') digits += s[i];
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  std::string out;
+  const int nd = (int)digits.size();
+  if (v == 0.0) {
+    out = "0";
+  } else if (e10 >= 0) {
+    if (nd <= e10 + 1) {
+      out = digits + std::string(e10 + 1 - nd, '0');
+    } else {
+      out = digits.substr(0, e10 + 1) + "." + digits.substr(e10 + 1);
+    }
+  } else {
+    out = "0." + std::string(-e10 - 1, '0') + digits;
+  }
+  return (neg ? "-" : "") + out;
+}
+
+namespace {
+
+int check_device(const Err& err) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return err.set(STOCHA_EDEVICE, "no HIP device available (%s)",
+                   e == hipSuccess ? "0 devices" : hipGetErrorName(e));
+  return STOCHA_OK;
+}
+
+// Runs a path-storing launch into host or device memory.
+int store_paths(int model, const void* mc, uint64_t seed, uint64_t block, uint64_t n_rows,
+                uint64_t base_rows, uint64_t steps, double* out, int where, const Err& err) {
+  if (int rc = check_device(err)) return rc;
+  hipStream_t st = globals().stream;
+  StreamDesc sd;
+  make_stream_desc(seed, block, &sd);
+  const size_t bytes = sizeof(double) * n_rows * (steps + 1);
+  if (where == STOCHA_DEVICE) {
+    if (int rc = launch_paths(model, mc, sd, n_rows, base_rows, steps, out, st, err)) return rc;
+    STOCHA_HIP(hipStreamSynchronize(st));
+    return STOCHA_OK;
+  }
+  DevBuf d;
+  if (int rc = d.alloc(bytes, err)) return rc;
+  if (int rc = launch_paths(model, mc, sd, n_rows, base_rows, steps, d.as<double>(), st, err))
+    return rc;
+  STOCHA_HIP(hipMemcpyAsync(out, d.p, bytes, hipMemcpyDeviceToHost, st));
+  STOCHA_HIP(hipStreamSynchronize(st));
+  return STOCHA_OK;
+}
+
+}  // namespace
+}  // namespace stocha
+
+using namespace stocha;
+
+extern "C" {
+
+const char* stocha_version(void) { return "stocha-gpu 0.1.0 (gfx950); reference stocha 1.7.1"; }
+
+int stocha_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int stocha_set_stream(void* s) {
+  std::lock_guard<std::mutex> lk(globals().mu);
+  globals().stream = (hipStream_t)s;
+  return STOCHA_OK;
+}
+
+int stocha_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(globals().mu);
+  Globals& g = globals();
+  if (on && !g.ev0) {
+    if (hipEventCreate(&g.ev0) != hipSuccess || hipEventCreate(&g.ev1) != hipSuccess)
+      return STOCHA_EDEVICE;
+  }
+  g.timing = on != 0;
+  return STOCHA_OK;
+}
+
+int stocha_last_kernel_ms(double* ms, uint64_t* launches) {
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (ms) *ms = globals().last_ms;
+  if (launches) *launches = globals().last_launches;
+  return STOCHA_OK;
+}
+
+// src/lib.rs:205-252
+int stocha_gbm(double s0, double mu, double sigma, double t, uint64_t steps, uint64_t n_paths,
+               uint64_t seed, int antithetic, double q, double* out, int where, char* errbuf,
+               size_t errlen) {
+  Err err{errbuf, errlen};
+  if (s0 <= 0.0) return err.set(STOCHA_EINVAL, "s0 must be positive");
+  if (sigma <= 0.0) return err.set(STOCHA_EINVAL, "sigma must be positive");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  if (!out) return STOCHA_OK;  // validate only
+  std::lock_guard<std::mutex> lk(globals().mu);
+  GbmConst c = gbm_const(s0, mu, sigma, t, steps, q);
+  const uint64_t base = antithetic ? (n_paths + 1) / 2 : n_paths;  // gbm.rs:48-52
+  return store_paths(kGbm, &c, seed, block_size(kGbm, steps), n_paths, base, steps, out, where,
+                     err);
+}
+
+// src/lib.rs:327-408
+int stocha_heston(double s0, double v0, double mu, double kappa, double theta, double xi,
+                  double rho, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
+                  const char* scheme, double* out, int where, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  if (s0 <= 0.0) return err.set(STOCHA_EINVAL, "s0 must be positive");
+  if (v0 < 0.0) return err.set(STOCHA_EINVAL, "v0 must be non-negative");
+  if (kappa <= 0.0) return err.set(STOCHA_EINVAL, "kappa must be positive");
+  if (theta <= 0.0) return err.set(STOCHA_EINVAL, "theta must be positive");
+  if (xi <= 0.0) return err.set(STOCHA_EINVAL, "xi must be positive");
+  if (rho < -1.0 || rho > 1.0) return err.set(STOCHA_EINVAL, "rho must be in [-1, 1]");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  const char* sch = scheme ? scheme : "euler";
+  if (strcmp(sch, "euler") != 0) {
+    if (strcmp(sch, "qe") == 0)
+      return err.set(STOCHA_ENOTIMPL,
+                     "scheme 'qe' is not part of the GPU hot path (only 'euler' is implemented)");
+    return err.set(STOCHA_EINVAL, "scheme must be 'euler' or 'qe'");
+  }
+  if (!out) return STOCHA_OK;  // validate only
+  std::lock_guard<std::mutex> lk(globals().mu);
+  const double p[7] = {s0, v0, mu, kappa, theta, xi, rho};
+  HestonConst c = heston_const(p, t, steps);
+  return store_paths(kHeston, &c, seed, block_size(kHeston, steps), n_paths, n_paths, steps, out,
+                     where, err);
+}
+
+// src/lib.rs:430-489
+int stocha_merton_jump_diffusion(double s0, double mu, double sigma, double lambda_, double mu_j,
+                                 double sigma_j, double t, uint64_t steps, uint64_t n_paths,
+                                 uint64_t seed, double* out, int where, char* errbuf,
+                                 size_t errlen) {
+  Err err{errbuf, errlen};
+  if (s0 <= 0.0) return err.set(STOCHA_EINVAL, "s0 must be positive");
+  if (sigma <= 0.0) return err.set(STOCHA_EINVAL, "sigma must be positive");
+  if (lambda_ < 0.0) return err.set(STOCHA_EINVAL, "lambda_ must be non-negative");
+  if (sigma_j < 0.0) return err.set(STOCHA_EINVAL, "sigma_j must be non-negative");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  if (!out) return STOCHA_OK;  // validate only
+  std::lock_guard<std::mutex> lk(globals().mu);
+  const double p[6] = {s0, mu, sigma, lambda_, mu_j, sigma_j};
+  MertonConst c = merton_const(p, t, steps);
+  return store_paths(kMerton, &c, seed, block_size(kMerton, steps), n_paths, n_paths, steps, out,
+                     where, err);
+}
+
+// src/lib.rs:759-827
+int stocha_multi_gbm(const double* s0, uint64_t n_s0, const double* mu, uint64_t n_mu,
+                     const double* sigma, uint64_t n_sigma, const double* corr,
+                     uint64_t corr_rows, uint64_t corr_cols, double t, uint64_t steps,
+                     uint64_t n_paths, uint64_t seed, int antithetic, double* out, int where,
+                     char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  const uint64_t n = n_s0;
+  if (n == 0) return err.set(STOCHA_EINVAL, "s0 must not be empty");
+  if (n_mu != n || n_sigma != n)
+    return err.set(STOCHA_EINVAL, "s0, mu, sigma must have the same length");
+  for (uint64_t i = 0; i < n; ++i)
+    if (s0[i] <= 0.0) return err.set(STOCHA_EINVAL, "all s0 must be positive");
+  for (uint64_t i = 0; i < n; ++i)
+    if (sigma[i] <= 0.0) return err.set(STOCHA_EINVAL, "all sigma must be positive");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  if (corr_rows != n || corr_cols != n)
+    return err.set(STOCHA_EINVAL, "corr must be (%llu, %llu), got (%llu, %llu)",
+                   (unsigned long long)n, (unsigned long long)n,
+                   (unsigned long long)corr_rows, (unsigned long long)corr_cols);
+  if (!out) {  // validate only (including the Cholesky factorisation)
+    std::vector<double> l(n * n);
+    return cholesky_host(corr, n, l.data(), err);
+  }
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  return run_multi_gbm_paths(s0, mu, sigma, corr, n, t, steps, n_paths, seed, antithetic, out,
+                             where, globals().stream, err);
+}
+
+// src/lib.rs:854-918
+static int lsmc_validate(double s0, double k, double sigma, double t, uint64_t steps,
+                         uint64_t n_paths, uint64_t poly_degree, const Err& err) {
+  if (s0 <= 0.0) return err.set(STOCHA_EINVAL, "s0 must be positive");
+  if (k <= 0.0) return err.set(STOCHA_EINVAL, "k must be positive");
+  if (sigma <= 0.0) return err.set(STOCHA_EINVAL, "sigma must be positive");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  if (n_paths < 2) return err.set(STOCHA_EINVAL, "n_paths must be at least 2");
+  if (poly_degree == 0 || poly_degree > 4)
+    return err.set(STOCHA_EINVAL, "poly_degree must be in [1, 4]");
+  return STOCHA_OK;
+}
+
+int stocha_lsmc_ex(double s0, double k, double r, double sigma, double t, uint64_t steps,
+                   uint64_t n_paths, int is_put, uint64_t poly_degree, uint64_t seed,
+                   int precision, double* price, double* std_err, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  if (int rc = lsmc_validate(s0, k, sigma, t, steps, n_paths, poly_degree, err)) return rc;
+  if (precision != STOCHA_FP64 && precision != STOCHA_FP32)
+    return err.set(STOCHA_EINVAL, "precision must be 32 or 64");
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  GbmConst c = gbm_const(s0, r, sigma, t, steps, 0.0);  // lsmc.rs:29-38
+  StreamDesc sd;
+  make_stream_desc(seed, block_size(kGbm, steps), &sd);
+  const double dt = t / (double)steps;
+  const double discount = exp(-r * dt);  // lsmc.rs:27
+  return run_lsmc(c, sd, n_paths, steps, k, discount, is_put ? 1 : 0, (int)poly_degree + 1,
+                  precision, price, std_err, globals().stream, err);
+}
+
+int stocha_lsmc_american_option(double s0, double k, double r, double sigma, double t,
+                                uint64_t steps, uint64_t n_paths, int is_put,
+                                uint64_t poly_degree, uint64_t seed, double* price,
+                                double* std_err, char* errbuf, size_t errlen) {
+  return stocha_lsmc_ex(s0, k, r, sigma, t, steps, n_paths, is_put, poly_degree, seed,
+                        STOCHA_FP64, price, std_err, errbuf, errlen);
+}
+
+// src/lib.rs:1177-1242 (+ parse_greeks :920-938)
+int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double strike,
+                           int is_call, uint64_t n_paths, uint64_t n_steps,
+                           const char* const* greeks, uint64_t n_greeks, uint64_t seed,
+                           double* values, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  if (s0 <= 0.0) return err.set(STOCHA_EINVAL, "s0 must be positive");
+  if (sigma <= 0.0) return err.set(STOCHA_EINVAL, "sigma must be positive");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (n_paths == 0 || n_steps == 0)
+    return err.set(STOCHA_EINVAL, "n_paths and n_steps must be positive");
+  // parse_greeks: every name must be known before the pathwise check runs
+  static const char* known[] = {"delta", "gamma", "vega", "theta", "rho"};
+  for (uint64_t i = 0; i < n_greeks; ++i) {
+    bool ok = false;
+    for (const char* kn : known) ok = ok || strcmp(greeks[i], kn) == 0;
+    if (!ok)
+      return err.set(STOCHA_EINVAL,
+                     "Unknown greek: '%s'. Supported: delta, gamma, vega, theta, rho", greeks[i]);
+  }
+  for (uint64_t i = 0; i < n_greeks; ++i) {
+    if (strcmp(greeks[i], "delta") != 0 && strcmp(greeks[i], "vega") != 0)
+      return err.set(STOCHA_EINVAL, "Pathwise method only supports 'delta' and 'vega', got '%s'",
+                     greeks[i]);
+  }
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  GbmConst c = gbm_const(s0, r, sigma, t, n_steps, 0.0);  // greeks.rs:368-375 (mu = r)
+  StreamDesc sd;
+  make_stream_desc(seed, block_size(kGbm, n_steps), &sd);
+  double dsum = 0.0, vsum = 0.0;
+  if (int rc = run_greeks_pathwise(c, sd, n_paths, n_steps, s0, r, sigma, t, strike, is_call,
+                                   &dsum, &vsum, globals().stream, err))
+    return rc;
+  const double discount = exp(-r * t);  // greeks.rs:377
+  const double n = (double)n_paths;
+  for (uint64_t i = 0; i < n_greeks; ++i)
+    values[i] = strcmp(greeks[i], "delta") == 0 ? discount * dsum / n : discount * vsum / n;
+  return STOCHA_OK;
+}
+
+int stocha_european_fused(int model, const double* params, uint64_t n_params, double t,
+                          uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
+                          const double* strikes, uint64_t n_strikes, double r, double* price,
+                          double* std_err, double* sums, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  static const uint64_t np[3] = {3, 7, 6};
+  if (model < 0 || model > 2) return err.set(STOCHA_EINVAL, "unknown model id %d", model);
+  if (n_params != np[model])
+    return err.set(STOCHA_EINVAL, "model %d takes %llu parameters, got %llu", model,
+                   (unsigned long long)np[model], (unsigned long long)n_params);
+  if (precision != STOCHA_FP64 && precision != STOCHA_FP32)
+    return err.set(STOCHA_EINVAL, "precision must be 32 or 64");
+  if (n_strikes == 0 || n_strikes > 8)
+    return err.set(STOCHA_EINVAL, "n_strikes must be in [1, 8]");
+  if (params[0] <= 0.0) return err.set(STOCHA_EINVAL, "s0 must be positive");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  GbmConst gc;
+  HestonConst hc;
+  MertonConst mc;
+  const void* cptr;
+  if (model == kGbm) {
+    gc = gbm_const(params[0], params[1], params[2], t, steps, 0.0);
+    cptr = &gc;
+  } else if (model == kHeston) {
+    hc = heston_const(params, t, steps);
+    cptr = &hc;
+  } else {
+    mc = merton_const(params, t, steps);
+    cptr = &mc;
+  }
+  StreamDesc sd;
+  make_stream_desc(seed, block_size(model, steps), &sd);
+  double h[16];
+  if (int rc = launch_fused(model, precision, cptr, sd, n_paths, steps, strikes, (int)n_strikes,
+                            h, globals().stream, err))
+    return rc;
+  const double disc = exp(-r * t);
+  const double n = (double)n_paths;
+  for (uint64_t k = 0; k < n_strikes; ++k) {
+    const double s1 = h[2 * k], s2 = h[2 * k + 1];
+    price[k] = disc * s1 / n;
+    double var = n > 1 ? (s2 - s1 * s1 / n) / (n - 1.0) : 0.0;
+    var = var > 0.0 ? var : 0.0;
+    std_err[k] = disc * sqrt(var / n);
+    if (sums) {
+      sums[2 * k] = s1;
+      sums[2 * k + 1] = s2;
+    }
+  }
+  return STOCHA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+// fused.hip -- terminal-value kernels that never store paths: European calls
+// priced in-register at up to 8 strikes, and the pathwise delta/vega sums of
+// greeks.rs:360-424.
+//
+// Each thread steps whole paths (grid-stride over path indices, every path on
+// its own block-split stream), evaluates the payoff on S_T and accumulates
+// FP64 sums.  Reduction: butterfly over the 64-lane wavefront, LDS across the
+// block's waves, one partial per block, then a single-block kernel sums the
+// partials in block order.  The grid size is a function of n_paths only, so
+// the summation order -- and the result -- is the same on every run.
+#include "host.hpp"
+
+namespace stocha {
+
+constexpr int kFusedBlock = 256;
+constexpr int kMaxStrikes = 8;
+constexpr int kMaxAcc = 2 * kMaxStrikes;
+
+struct CallPayoff {
+  double k[kMaxStrikes];
+  int n;
+  __device__ __forceinline__ int n_acc() const { return 2 * n; }
+  __device__ __forceinline__ void add(double st, double* acc) const {
+#pragma unroll
+    for (int i = 0; i < kMaxStrikes; ++i) {
+      if (i < n) {
+        double pay = st - k[i];
+        pay = pay > 0.0 ? pay : 0.0;
+        acc[2 * i] += pay;
+        acc[2 * i + 1] += pay * pay;
+      }
+    }
+  }
+};
+
+// greeks.rs:388-418: per-path delta and vega terms for a call or put.
+struct PathwisePayoff {
+  double s0, strike, drift_t, sigma;  // drift_t = (r + 0.5*sigma*sigma)*t
+  int is_call;
+  __device__ __forceinline__ int n_acc() const { return 2; }
+  __device__ __forceinline__ void add(double s_t, double* acc) const {
+    const bool in_money = is_call ? s_t > strike : s_t < strike;
+    if (in_money) {
+      const double sign = is_call ? 1.0 : -1.0;
+      acc[0] += sign * s_t / s0;
+      const double log_return = log(s_t / s0);
+      acc[1] += sign * s_t * (log_return - drift_t) / sigma;
+    }
+  }
+};
+
+template <class Model, class Const, bool kFp32>
+__device__ __forceinline__ double terminal_value(const Const& mc, const StreamDesc& sd,
+                                                 uint64_t path, uint64_t steps,
+                                                 const ZigLds* zig) {
+  Pcg g{stream_at(&sd, path)};
+  const U128 inc = sd.inc;
+  Model m(mc, false);
+#pragma unroll 1
+  for (uint64_t j = 0; j < steps; ++j) (void)m.step(g, inc, zig);
+  if constexpr (kFp32) {
+    return mc.s0 * exp((double)m.ls);
+  } else {
+    return m.s;
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <class Model, class Const, bool kFp32, class Payoff>
+__global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
+    const Const mc, const StreamDesc sd, const uint64_t n_paths, const uint64_t steps,
+    const Payoff pay, double* __restrict__ partials) {
+  __shared__ ZigLds zig;
+  __shared__ double red[kFusedBlock / 64][kMaxAcc];
+  zig_load(&zig);
+  __syncthreads();
+
+  double acc[kMaxAcc];
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) acc[i] = 0.0;
+
+  const uint64_t stride = (uint64_t)gridDim.x * kFusedBlock;
+  for (uint64_t p = (uint64_t)blockIdx.x * kFusedBlock + threadIdx.x; p < n_paths; p += stride) {
+    const double st = terminal_value<Model, Const, kFp32>(mc, sd, p, steps, &zig);
+    pay.add(st, acc);
+  }
+
+  const int na = pay.n_acc();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) {
+    if (i < na) {
+      const double w = wave_sum(acc[i]);
+      if (lane == 0) red[wave][i] = w;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < na) {
+    double s = 0.0;
+    for (int w = 0; w < kFusedBlock / 64; ++w) s += red[w][threadIdx.x];
+    partials[(uint64_t)blockIdx.x * na + threadIdx.x] = s;
+  }
+}
+
+// Sums partials[b][i] over b in block order; one thread per accumulator
+// slot strides over blocks, then a fixed tree over the 256 threads.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ partials,
+                                                              const int n_blocks, const int na,
+                                                              double* __restrict__ out) {
+  __shared__ double buf[256];
+  for (int i = 0; i < na; ++i) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < n_blocks; b += 256) s += partials[(uint64_t)b * na + i];
+    buf[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = buf[0];
+    __syncthreads();
+  }
+}
+
+static unsigned fused_grid(uint64_t n_paths) {
+  // One path per thread up to 4096 blocks (16 resident waves on each of the
+  // 256 CUs, twice over); larger runs loop.  Depends on n_paths only.
+  uint64_t b = (n_paths + kFusedBlock - 1) / kFusedBlock;
+  if (b > 4096) b = 4096;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
+template <class Model, class Const, bool kFp32, class Payoff>
+static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
+                        const Payoff& pay, int na, double* h_out, hipStream_t st,
+                        const Err& err) {
+  const unsigned grid = fused_grid(n_paths);
+  DevBuf part, out;
+  if (int rc = part.alloc(sizeof(double) * grid * na, err)) return rc;
+  if (int rc = out.alloc(sizeof(double) * na, err)) return rc;
+  KernelTimer timer(st);
+  timer.start();
+  terminal_payoff_kernel<Model, Const, kFp32, Payoff>
+      <<<grid, kFusedBlock, 0, st>>>(mc, sd, n_paths, steps, pay, part.as<double>());
+  STOCHA_HIP(hipGetLastError());
+  timer.stop(1);
+  reduce_partials_kernel<<<1, 256, 0, st>>>(part.as<double>(), (int)grid, na, out.as<double>());
+  STOCHA_HIP(hipGetLastError());
+  STOCHA_HIP(hipMemcpyAsync(h_out, out.p, sizeof(double) * na, hipMemcpyDeviceToHost, st));
+  STOCHA_HIP(hipStreamSynchronize(st));
+  return STOCHA_OK;
+}
+
+int launch_fused(int model, int precision, const void* mconst, const StreamDesc& sd,
+                 uint64_t n_paths, uint64_t steps, const double* strikes, int n_strikes,
+                 double* h_sums, hipStream_t st, const Err& err) {
+  CallPayoff pay{};
+  pay.n = n_strikes;
+  for (int i = 0; i < n_strikes; ++i) pay.k[i] = strikes[i];
+  const int na = 2 * n_strikes;
+  const bool f32 = precision == STOCHA_FP32;
+  switch (model) {
+    case kGbm: {
+      const GbmConst& c = *(const GbmConst*)mconst;
+      return f32 ? run_terminal<Gbm32, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, st, err)
+                 : run_terminal<Gbm64, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, st, err);
+    }
+    case kHeston: {
+      const HestonConst& c = *(const HestonConst*)mconst;
+      return f32 ? run_terminal<Heston32, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, st, err)
+                 : run_terminal<Heston64, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, st, err);
+    }
+    case kMerton: {
+      const MertonConst& c = *(const MertonConst*)mconst;
+      return f32 ? run_terminal<Merton32, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, st, err)
+                 : run_terminal<Merton64, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, st, err);
+    }
+    default:
+      return err.set(STOCHA_EINVAL, "unknown model id %d", model);
+  }
+}
+
+int run_greeks_pathwise(const GbmConst& gc, const StreamDesc& sd, uint64_t n_paths,
+                        uint64_t steps, double s0, double r, double sigma, double t,
+                        double strike, int is_call, double* delta_sum, double* vega_sum,
+                        hipStream_t st, const Err& err) {
+  PathwisePayoff pay{};
+  pay.s0 = s0;
+  pay.strike = strike;
+  pay.drift_t = (r + 0.5 * sigma * sigma) * t;  // greeks.rs:411
+  pay.sigma = sigma;
+  pay.is_call = is_call;
+  double h[2];
+  int rc = run_terminal<Gbm64, GbmConst, false>(gc, sd, n_paths, steps, pay, 2, h, st, err);
+  *delta_sum = h[0];
+  *vega_sum = h[1];
+  return rc;
+}
+
+}  // namespace stocha

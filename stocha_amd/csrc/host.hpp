@@ -1,0 +1,164 @@
+// host.hpp -- host-side plumbing shared by the entry points: error strings,
+// stream-table setup (128-bit jump-ahead on the host), device buffers and the
+// kernel-time probe.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <mutex>
+
+#include "models.hpp"
+#include "stocha_gpu.h"
+
+namespace stocha {
+
+// ---------------------------------------------------------------- errors --
+struct Err {
+  char* buf;
+  size_t len;
+  int set(int code, const char* fmt, ...) const __attribute__((format(printf, 3, 4))) {
+    if (buf && len) {
+      va_list ap;
+      va_start(ap, fmt);
+      vsnprintf(buf, len, fmt, ap);
+      va_end(ap);
+    }
+    return code;
+  }
+};
+
+#define STOCHA_HIP(call)                                                                  \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return err.set(e_ == hipErrorOutOfMemory ? STOCHA_ENOMEM : STOCHA_EDEVICE,          \
+                     "HIP error %s at %s:%d (%s)", hipGetErrorName(e_), __FILE__, __LINE__, \
+                     #call);                                                              \
+  } while (0)
+
+// ------------------------------------------------------- global state --
+struct Globals {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  double last_ms = 0.0;
+  uint64_t last_launches = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+Globals& globals();
+
+// ------------------------------------------------------ device buffers --
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n, const Err& err) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return err.set(STOCHA_ENOMEM, "device allocation of %zu bytes failed (%s)", n,
+                     hipGetErrorName(e));
+    }
+    bytes = n;
+    return STOCHA_OK;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// ------------------------------------------------- kernel-time probe --
+// Records events around the dominant kernel(s) of a call on the work stream.
+struct KernelTimer {
+  bool on;
+  hipStream_t s;
+  explicit KernelTimer(hipStream_t st) : on(globals().timing), s(st) {}
+  void start() {
+    if (on) (void)hipEventRecord(globals().ev0, s);
+  }
+  void stop(uint64_t launches) {
+    if (!on) return;
+    (void)hipEventRecord(globals().ev1, s);
+    (void)hipEventSynchronize(globals().ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, globals().ev0, globals().ev1);
+    globals().last_ms = ms;
+    globals().last_launches = launches;
+  }
+};
+
+// ------------------------------------------------------- stream tables --
+// Affine map of the LCG after `steps` steps: s -> a*s + c (mod 2^128).
+typedef unsigned __int128 u128h;
+inline void lcg_power(u128h inc, u128h steps, u128h* a_out, u128h* c_out) {
+  u128h acc_a = 1, acc_c = 0, cur_a = kPcgMul, cur_c = inc;
+  while (steps) {
+    if (steps & 1) {
+      acc_a = acc_a * cur_a;
+      acc_c = acc_c * cur_a + cur_c;
+    }
+    cur_c = (cur_a + 1) * cur_c;
+    cur_a = cur_a * cur_a;
+    steps >>= 1;
+  }
+  *a_out = acc_a;
+  *c_out = acc_c;
+}
+
+inline U128 to_u128(u128h v) { return U128{(uint64_t)v, (uint64_t)(v >> 64)}; }
+
+// Pcg64Dxsm::new(seed) (pcg64.rs:23-30) and jumps of block*2^k steps.
+inline void make_stream_desc(uint64_t seed, uint64_t block, StreamDesc* d) {
+  u128h s = seed;
+  u128h stream = s + (((u128h)0xda3e39cb94b95bdbull << 64) | 0x7ef55da5c0374f5aull);
+  u128h inc = (stream << 1) | 1;
+  u128h state = s + inc;
+  state = state * (u128h)kPcgMul + inc;
+  d->base = to_u128(state);
+  d->inc = to_u128(inc);
+  u128h a, c;
+  lcg_power(inc, (u128h)block, &a, &c);
+  for (int k = 0; k < kJumpBits; ++k) {
+    d->jump[k].a = to_u128(a);
+    d->jump[k].c = to_u128(c);
+    // square the map: (a, c) o (a, c) = (a^2, a*c + c)
+    c = a * c + c;
+    a = a * a;
+  }
+}
+
+// ------------------------------------------------------ constant setup --
+// The expressions below are the reference's, evaluated on the host in
+// binary64 (this file is compiled with -ffp-contract=off).
+GbmConst gbm_const(double s0, double mu, double sigma, double t, uint64_t steps, double q);
+HestonConst heston_const(const double* p7, double t, uint64_t steps);
+MertonConst merton_const(const double* p6, double t, uint64_t steps);
+
+// ------------------------------------------------------------ launchers --
+// paths.hip
+int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n_rows,
+                 uint64_t base_rows, uint64_t steps, double* d_out, hipStream_t st,
+                 const Err& err);
+// fused.hip
+int launch_fused(int model, int precision, const void* mconst, const StreamDesc& sd,
+                 uint64_t n_paths, uint64_t steps, const double* strikes, int n_strikes,
+                 double* h_sums, hipStream_t st, const Err& err);
+// lsmc.hip
+int run_lsmc(const GbmConst& gc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
+             double k, double discount, int is_put, int ncols, int precision, double* price,
+             double* std_err, hipStream_t st, const Err& err);
+// greeks.hip
+int run_greeks_pathwise(const GbmConst& gc, const StreamDesc& sd, uint64_t n_paths,
+                        uint64_t steps, double s0, double r, double sigma, double t,
+                        double strike, int is_call, double* delta_sum, double* vega_sum,
+                        hipStream_t st, const Err& err);
+
+}  // namespace stocha

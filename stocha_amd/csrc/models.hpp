@@ -1,0 +1,147 @@
+// models.hpp -- per-path steppers for GBM, Heston (full-truncation Euler)
+// and Merton jump-diffusion, in the reference's evaluation order (FP64) and in
+// a log-space FP32 form.
+//
+// FP64 forms restate, operation for operation (this file is compiled with
+// -ffp-contract=off, so a*b+c is two roundings as in Rust):
+//   GBM     gbm.rs:43-45 (constants), :65-73 (step)
+//   Heston  heston.rs:45-47 (constants), :59-71 (step)
+//   Merton  jump_diffusion.rs:49-64 (constants), :74-89 (step)
+// The host computes the per-launch constants (drift, diffusion, ...) with the
+// same expressions, so device and reference agree bit-for-bit up to the libm
+// exp used per step.
+//
+// FP32 forms step the log-price: log S += increment, with the increment
+// computed in FP32 (explicit fmaf), and exponentiate once when S is needed.
+// They draw exactly the same normals/uniforms, in the same order, as the FP64
+// forms (see rng.hpp), so their error against the FP64 result is rounding
+// only.
+#pragma once
+#include "rng.hpp"
+
+namespace stocha {
+
+enum ModelKind : int { kGbm = 0, kHeston = 1, kMerton = 2 };
+
+struct GbmConst {
+  double s0, drift, diffusion;
+  float drift_f, diffusion_f;
+};
+
+struct HestonConst {
+  double s0, v0, mu, kappa, theta, xi, rho, rho_comp, dt, sqrt_dt;
+  // FP32 log-space constants
+  float v0_f, rho_f, rho_comp_f, kappa_dt_f, theta_f, xi_sqdt_f, mu_dt_f, half_dt_f, sqrt_dt_f;
+};
+
+struct MertonConst {
+  double s0, drift, diffusion, jump_prob, mu_j, sigma_j;
+  float drift_f, diffusion_f, mu_j_f, sigma_j_f;
+};
+
+// Stream budget per path (reference block sizes).
+__host__ __device__ inline uint64_t block_size(int model, uint64_t steps) {
+  return model == kGbm ? (steps + 1024) * 2 : model == kHeston ? (steps + 1024) * 6 : (steps + 1024) * 8;
+}
+
+// ---------------------------------------------------------------- FP64 ----
+
+struct Gbm64 {
+  const GbmConst c;
+  double s;
+  bool neg;
+  __device__ __forceinline__ Gbm64(const GbmConst& k, bool negate) : c(k), s(k.s0), neg(negate) {}
+  __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
+    double n = zig_normal64(g, inc, z);
+    n = neg ? -n : n;
+    s *= exp(c.drift + c.diffusion * n);
+    return s;
+  }
+};
+
+struct Heston64 {
+  const HestonConst c;
+  double s, v;
+  __device__ __forceinline__ Heston64(const HestonConst& k, bool) : c(k), s(k.s0), v(k.v0) {}
+  __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
+    double dw1 = zig_normal64(g, inc, z);
+    double dz = zig_normal64(g, inc, z);
+    double dw2 = c.rho * dw1 + c.rho_comp * dz;
+    double v_plus = fmax(v, 0.0);
+    double sqrt_v = sqrt(v_plus);
+    v += c.kappa * (c.theta - v_plus) * c.dt + c.xi * sqrt_v * c.sqrt_dt * dw2;
+    s *= exp((c.mu - 0.5 * v_plus) * c.dt + sqrt_v * c.sqrt_dt * dw1);
+    return s;
+  }
+};
+
+struct Merton64 {
+  const MertonConst c;
+  double s;
+  __device__ __forceinline__ Merton64(const MertonConst& k, bool) : c(k), s(k.s0) {}
+  __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
+    double zz = zig_normal64(g, inc, z);
+    double diffusion_return = c.drift + c.diffusion * zz;
+    double u = g.next_f64(inc);
+    double jump_return = 0.0;
+    if (u < c.jump_prob) {
+      double zj = zig_normal64(g, inc, z);
+      jump_return = c.mu_j + c.sigma_j * zj;
+    }
+    s *= exp(diffusion_return + jump_return);
+    return s;
+  }
+};
+
+// ---------------------------------------------------------------- FP32 ----
+// step() returns the log-price increment's running sum; s0 * exp(ls) is S.
+
+struct Gbm32 {
+  const GbmConst c;
+  float ls;
+  bool neg;
+  __device__ __forceinline__ Gbm32(const GbmConst& k, bool negate) : c(k), ls(0.f), neg(negate) {}
+  __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
+    float n = zig_normal32(g, inc, z);
+    n = neg ? -n : n;
+    ls += fmaf(c.diffusion_f, n, c.drift_f);
+    return ls;
+  }
+};
+
+struct Heston32 {
+  const HestonConst c;
+  float ls, v;
+  __device__ __forceinline__ Heston32(const HestonConst& k, bool) : c(k), ls(0.f), v(k.v0_f) {}
+  __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
+    float dw1 = zig_normal32(g, inc, z);
+    float dz = zig_normal32(g, inc, z);
+    float dw2 = fmaf(c.rho_f, dw1, c.rho_comp_f * dz);
+    float v_plus = fmaxf(v, 0.f);
+    float sqrt_v = __builtin_sqrtf(v_plus);
+    // v += kappa*(theta - v+)*dt + xi*sqrt(v+)*sqrt(dt)*dw2
+    v = fmaf(c.kappa_dt_f, c.theta_f - v_plus, fmaf(c.xi_sqdt_f * sqrt_v, dw2, v));
+    // log S += (mu - v+/2)*dt + sqrt(v+)*sqrt(dt)*dw1
+    ls += fmaf(-c.half_dt_f, v_plus, fmaf(sqrt_v * c.sqrt_dt_f, dw1, c.mu_dt_f));
+    return ls;
+  }
+};
+
+struct Merton32 {
+  const MertonConst c;
+  float ls;
+  __device__ __forceinline__ Merton32(const MertonConst& k, bool) : c(k), ls(0.f) {}
+  __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
+    float zz = zig_normal32(g, inc, z);
+    float r = fmaf(c.diffusion_f, zz, c.drift_f);
+    double u = g.next_f64(inc);  // exact, same comparison as FP64
+    if (u < c.jump_prob) {
+      float zj = zig_normal32(g, inc, z);
+      r += fmaf(c.sigma_j_f, zj, c.mu_j_f);
+    }
+    ls += r;
+    return ls;
+  }
+};
+
+}  // namespace stocha
