@@ -54,6 +54,13 @@ extern "C" {
 #define STOCHA_FP64 64
 #define STOCHA_FP32 32
 
+/* Schedules of the in-register forms (same draws, same values):
+ *   FEED  every lane draws one u64 per iteration and advances its own
+ *         ziggurat/step state machine (no wave-wide waits on slow lanes)
+ *   LOOP  the reference's control flow: one ziggurat loop per normal */
+#define STOCHA_SCHEDULE_FEED 0
+#define STOCHA_SCHEDULE_LOOP 1
+
 /* Model ids for the fused, batched and Greeks entry points. */
 #define STOCHA_MODEL_GBM 0
 #define STOCHA_MODEL_HESTON 1
@@ -136,14 +143,16 @@ int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double s
  *   Merton {s0, mu, sigma, lambda, mu_j, sigma_j} (n_params 6)
  * precision: STOCHA_FP64 (reference arithmetic, s *= exp(.) per step) or
  * STOCHA_FP32 (log-space FP32 stepping, FP64 payoff sums).  Draws are the
- * reference's block-split streams in both cases.  price[k] =
+ * reference's block-split streams in both cases.  schedule:
+ * STOCHA_SCHEDULE_FEED or STOCHA_SCHEDULE_LOOP (identical results).  price[k] =
  * exp(-r t) * mean payoff, std_err[k] = exp(-r t) * sqrt(var_{n-1} / n);
  * sums (optional, 2*n_strikes) receives the raw FP64 sum and sum of squares of
  * the undiscounted payoffs per strike. */
 int stocha_european_fused(int model, const double *params, uint64_t n_params, double t,
                           uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
-                          const double *strikes, uint64_t n_strikes, double r, double *price,
-                          double *std_err, double *sums, char *err, size_t errlen);
+                          int schedule, const double *strikes, uint64_t n_strikes, double r,
+                          double *price, double *std_err, double *sums, char *err,
+                          size_t errlen);
 
 /* LSMC with a choice of path precision (FP32 paths: compensated FP64
  * moment sums, FP64 solve).  Same validation as stocha_lsmc_american_option. */

@@ -113,16 +113,21 @@ _MODELS = {"gbm": (_lib.MODEL_GBM, ("s0", "mu", "sigma")),
 
 def european_price(model: str, params: dict, t: float, steps: int, n_paths: int,
                    strikes: Sequence[float], r: float, seed: int = 42,
-                   precision: int = 64) -> dict:
+                   precision: int = 64, schedule: str = "feed") -> dict:
     """European calls at up to 8 strikes, priced in-register on the GPU.
 
     model: "gbm" | "heston" | "merton"; params holds the model's parameters
     under the names of the corresponding path function.  precision 64 steps
     with the reference's FP64 arithmetic; 32 steps the log-price in FP32 with
     FP64 payoff sums.  Both use the reference's block-split streams, so they
-    price the same paths.  Returns {"price": array, "std_err": array,
-    "sum": array, "sumsq": array} (one entry per strike).
+    price the same paths.  schedule "feed" (per-lane state machine, default)
+    and "loop" (the reference's per-normal loop) give identical results.
+    Returns {"price": array, "std_err": array, "sum": array, "sumsq": array}
+    (one entry per strike).
     """
+    if schedule not in ("feed", "loop"):
+        raise ValueError("schedule must be 'feed' or 'loop'")
+    sched = _lib.STOCHA_SCHEDULE_FEED if schedule == "feed" else _lib.STOCHA_SCHEDULE_LOOP
     if model not in _MODELS:
         raise ValueError(f"Unknown model: '{model}'. Supported: gbm, heston, merton")
     mid, names = _MODELS[model]
@@ -139,7 +144,8 @@ def european_price(model: str, params: dict, t: float, steps: int, n_paths: int,
     se = np.zeros(max(ns, 1))
     sums = np.zeros(2 * max(ns, 1))
     rc = lib.stocha_european_fused(mid, _S._pd(p), len(p), _S._f64(t, "t"), steps, n_paths,
-                                   seed, int(precision), _S._pd(k), ns, _S._f64(r, "r"),
+                                   seed, int(precision), sched, _S._pd(k), ns,
+                                   _S._f64(r, "r"),
                                    _S._pd(price), _S._pd(se), _S._pd(sums), eb.buf, eb.n)
     _lib.check(rc, eb)
     return {"price": price[:ns], "std_err": se[:ns], "sum": sums[0:2 * ns:2],

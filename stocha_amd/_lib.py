@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libstocha_gpu.so")
 STOCHA_OK, STOCHA_EINVAL, STOCHA_EDEVICE, STOCHA_ENOMEM, STOCHA_ENOTIMPL = 0, 1, 2, 3, 4
 STOCHA_HOST, STOCHA_DEVICE = 0, 1
 STOCHA_FP64, STOCHA_FP32 = 64, 32
+STOCHA_SCHEDULE_FEED, STOCHA_SCHEDULE_LOOP = 0, 1
 MODEL_GBM, MODEL_HESTON, MODEL_MERTON = 0, 1, 2
 
 _d = C.c_double
@@ -51,8 +52,8 @@ _SIGS = {
                                     C.c_int),
     "stocha_greeks_pathwise": ([_d] * 5 + [_i, _u, _u, C.POINTER(C.c_char_p), _u, _u, _pd, _ps,
                                            _sz], C.c_int),
-    "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _pd, _u, _d, _pd, _pd, _pd, _ps,
-                               _sz], C.c_int),
+    "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _pd, _pd, _pd,
+                               _ps, _sz], C.c_int),
     "stocha_lsmc_ex": ([_d] * 5 + [_u, _u, _i, _u, _u, _i, _pd, _pd, _ps, _sz], C.c_int),
     "stocha_stream_u64": ([_u, _u, _u, _u, _u, _pu, _ps, _sz], C.c_int),
     "stocha_stream_normals": ([_u, _u, _u, _u, _u, _i, _pd, _pu, _ps, _sz], C.c_int),

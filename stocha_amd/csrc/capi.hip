@@ -369,8 +369,9 @@ int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double s
 
 int stocha_european_fused(int model, const double* params, uint64_t n_params, double t,
                           uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
-                          const double* strikes, uint64_t n_strikes, double r, double* price,
-                          double* std_err, double* sums, char* errbuf, size_t errlen) {
+                          int schedule, const double* strikes, uint64_t n_strikes, double r,
+                          double* price, double* std_err, double* sums, char* errbuf,
+                          size_t errlen) {
   Err err{errbuf, errlen};
   static const uint64_t np[3] = {3, 7, 6};
   if (model < 0 || model > 2) return err.set(STOCHA_EINVAL, "unknown model id %d", model);
@@ -379,6 +380,8 @@ int stocha_european_fused(int model, const double* params, uint64_t n_params, do
                    (unsigned long long)np[model], (unsigned long long)n_params);
   if (precision != STOCHA_FP64 && precision != STOCHA_FP32)
     return err.set(STOCHA_EINVAL, "precision must be 32 or 64");
+  if (schedule != STOCHA_SCHEDULE_FEED && schedule != STOCHA_SCHEDULE_LOOP)
+    return err.set(STOCHA_EINVAL, "schedule must be 0 (feed) or 1 (loop)");
   if (n_strikes == 0 || n_strikes > 8)
     return err.set(STOCHA_EINVAL, "n_strikes must be in [1, 8]");
   if (params[0] <= 0.0) return err.set(STOCHA_EINVAL, "s0 must be positive");
@@ -404,8 +407,8 @@ int stocha_european_fused(int model, const double* params, uint64_t n_params, do
   StreamDesc sd;
   make_stream_desc(seed, block_size(model, steps), &sd);
   double h[16];
-  if (int rc = launch_fused(model, precision, cptr, sd, n_paths, steps, strikes, (int)n_strikes,
-                            h, globals().stream, err))
+  if (int rc = launch_fused(model, precision, schedule, cptr, sd, n_paths, steps, strikes,
+                            (int)n_strikes, h, globals().stream, err))
     return rc;
   const double disc = exp(-r * t);
   const double n = (double)n_paths;

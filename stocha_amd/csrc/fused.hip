@@ -8,7 +8,15 @@
 // block's waves, one partial per block, then a single-block kernel sums the
 // partials in block order.  The grid size is a function of n_paths only, so
 // the summation order -- and the result -- is the same on every run.
+//
+// Two schedules draw the same numbers:
+//   kScheduleLoop  per-normal ziggurat loop (the reference's control flow)
+//   kScheduleFeed  flat per-lane state machine (zigsm.hpp): every iteration
+//                  draws one u64 on every lane; a lane that finishes a path
+//                  accumulates its payoff and starts its next path inside the
+//                  same loop, so lanes never wait on each other.
 #include "host.hpp"
+#include "zigsm.hpp"
 
 namespace stocha {
 
@@ -65,6 +73,93 @@ __device__ __forceinline__ double terminal_value(const Const& mc, const StreamDe
   }
 }
 
+// ------------------------------------------------ state-machine drivers --
+// feed(u) consumes one draw; returns true when the path's last step is done.
+template <class M>
+struct GbmDrv {
+  using Real = typename ZigFeed<M::kFp32>::Real;
+  M m;
+  ZigFeed<M::kFp32> zf;
+  uint32_t done = 0;
+  template <class C>
+  __device__ __forceinline__ explicit GbmDrv(const C& c) : m(c, false) {}
+  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
+                                       uint32_t steps) {
+    Real x;
+    if (zf.feed(u, g, inc, z, x)) {
+      m.apply(x);
+      return ++done == steps;
+    }
+    return false;
+  }
+};
+
+template <class M>
+struct HestonDrv {
+  using Real = typename ZigFeed<M::kFp32>::Real;
+  M m;
+  ZigFeed<M::kFp32> zf;
+  uint32_t done = 0;
+  bool have = false;
+  Real dw1 = 0;
+  template <class C>
+  __device__ __forceinline__ explicit HestonDrv(const C& c) : m(c, false) {}
+  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
+                                       uint32_t steps) {
+    Real x;
+    if (zf.feed(u, g, inc, z, x)) {
+      if (!have) {
+        dw1 = x;
+        have = true;
+        return false;
+      }
+      have = false;
+      m.apply(dw1, x);
+      return ++done == steps;
+    }
+    return false;
+  }
+};
+
+// Merton's per-step grammar: normal z, then one uniform (always), then a
+// jump normal when the uniform is below lambda*dt (jump_diffusion.rs:75-85).
+template <class M>
+struct MertonDrv {
+  using Real = typename ZigFeed<M::kFp32>::Real;
+  M m;
+  ZigFeed<M::kFp32> zf;
+  uint32_t done = 0;
+  int phase = 0;  // 0: drawing z, 1: next draw is the jump uniform, 2: drawing z_j
+  Real zz = 0;
+  template <class C>
+  __device__ __forceinline__ explicit MertonDrv(const C& c) : m(c, false) {}
+  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
+                                       uint32_t steps) {
+    if (phase == 1) {
+      const double uf = (double)(u >> 11) * 0x1p-53;
+      if (uf < m.c.jump_prob) {
+        phase = 2;
+        return false;
+      }
+      phase = 0;
+      m.apply(zz, false, (Real)0);
+      return ++done == steps;
+    }
+    Real x;
+    if (zf.feed(u, g, inc, z, x)) {
+      if (phase == 0) {
+        zz = x;
+        phase = 1;
+        return false;
+      }
+      phase = 0;
+      m.apply(zz, true, x);
+      return ++done == steps;
+    }
+    return false;
+  }
+};
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -88,6 +183,55 @@ __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
   for (uint64_t p = (uint64_t)blockIdx.x * kFusedBlock + threadIdx.x; p < n_paths; p += stride) {
     const double st = terminal_value<Model, Const, kFp32>(mc, sd, p, steps, &zig);
     pay.add(st, acc);
+  }
+
+  const int na = pay.n_acc();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) {
+    if (i < na) {
+      const double w = wave_sum(acc[i]);
+      if (lane == 0) red[wave][i] = w;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < na) {
+    double s = 0.0;
+    for (int w = 0; w < kFusedBlock / 64; ++w) s += red[w][threadIdx.x];
+    partials[(uint64_t)blockIdx.x * na + threadIdx.x] = s;
+  }
+}
+
+// Flat state-machine kernel: one draw per lane per iteration.
+template <class Drv, class Const, class Payoff>
+__global__ __launch_bounds__(kFusedBlock) void terminal_feed_kernel(
+    const Const mc, const StreamDesc sd, const uint64_t n_paths, const uint32_t steps,
+    const Payoff pay, double* __restrict__ partials) {
+  __shared__ ZigLds zig;
+  __shared__ double red[kFusedBlock / 64][kMaxAcc];
+  zig_load(&zig);
+  __syncthreads();
+
+  double acc[kMaxAcc];
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) acc[i] = 0.0;
+
+  const uint64_t stride = (uint64_t)gridDim.x * kFusedBlock;
+  uint64_t p = (uint64_t)blockIdx.x * kFusedBlock + threadIdx.x;
+  if (p < n_paths) {
+    const U128 inc = sd.inc;
+    Pcg g{stream_at(&sd, p)};
+    Drv d(mc);
+    for (;;) {
+      const uint64_t u = g.next(inc);
+      if (d.feed(u, g, inc, &zig, steps)) {
+        pay.add(d.m.terminal(), acc);
+        p += stride;
+        if (p >= n_paths) break;
+        g.s = stream_at(&sd, p);
+        d = Drv(mc);
+      }
+    }
   }
 
   const int na = pay.n_acc();
@@ -136,18 +280,23 @@ static unsigned fused_grid(uint64_t n_paths) {
   return (unsigned)b;
 }
 
-template <class Model, class Const, bool kFp32, class Payoff>
+template <class Model, class Drv, class Const, bool kFp32, class Payoff>
 static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
-                        const Payoff& pay, int na, double* h_out, hipStream_t st,
+                        const Payoff& pay, int na, double* h_out, int schedule, hipStream_t st,
                         const Err& err) {
+  if (steps > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps must be below 2^32");
   const unsigned grid = fused_grid(n_paths);
   DevBuf part, out;
   if (int rc = part.alloc(sizeof(double) * grid * na, err)) return rc;
   if (int rc = out.alloc(sizeof(double) * na, err)) return rc;
   KernelTimer timer(st);
   timer.start();
-  terminal_payoff_kernel<Model, Const, kFp32, Payoff>
-      <<<grid, kFusedBlock, 0, st>>>(mc, sd, n_paths, steps, pay, part.as<double>());
+  if (schedule == kScheduleLoop)
+    terminal_payoff_kernel<Model, Const, kFp32, Payoff>
+        <<<grid, kFusedBlock, 0, st>>>(mc, sd, n_paths, steps, pay, part.as<double>());
+  else
+    terminal_feed_kernel<Drv, Const, Payoff>
+        <<<grid, kFusedBlock, 0, st>>>(mc, sd, n_paths, (uint32_t)steps, pay, part.as<double>());
   STOCHA_HIP(hipGetLastError());
   timer.stop(1);
   reduce_partials_kernel<<<1, 256, 0, st>>>(part.as<double>(), (int)grid, na, out.as<double>());
@@ -157,9 +306,9 @@ static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths,
   return STOCHA_OK;
 }
 
-int launch_fused(int model, int precision, const void* mconst, const StreamDesc& sd,
-                 uint64_t n_paths, uint64_t steps, const double* strikes, int n_strikes,
-                 double* h_sums, hipStream_t st, const Err& err) {
+int launch_fused(int model, int precision, int schedule, const void* mconst,
+                 const StreamDesc& sd, uint64_t n_paths, uint64_t steps, const double* strikes,
+                 int n_strikes, double* h_sums, hipStream_t st, const Err& err) {
   CallPayoff pay{};
   pay.n = n_strikes;
   for (int i = 0; i < n_strikes; ++i) pay.k[i] = strikes[i];
@@ -168,18 +317,18 @@ int launch_fused(int model, int precision, const void* mconst, const StreamDesc&
   switch (model) {
     case kGbm: {
       const GbmConst& c = *(const GbmConst*)mconst;
-      return f32 ? run_terminal<Gbm32, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, st, err)
-                 : run_terminal<Gbm64, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, st, err);
+      return f32 ? run_terminal<Gbm32, GbmDrv<Gbm32>, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+                 : run_terminal<Gbm64, GbmDrv<Gbm64>, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     case kHeston: {
       const HestonConst& c = *(const HestonConst*)mconst;
-      return f32 ? run_terminal<Heston32, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, st, err)
-                 : run_terminal<Heston64, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, st, err);
+      return f32 ? run_terminal<Heston32, HestonDrv<Heston32>, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+                 : run_terminal<Heston64, HestonDrv<Heston64>, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     case kMerton: {
       const MertonConst& c = *(const MertonConst*)mconst;
-      return f32 ? run_terminal<Merton32, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, st, err)
-                 : run_terminal<Merton64, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, st, err);
+      return f32 ? run_terminal<Merton32, MertonDrv<Merton32>, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+                 : run_terminal<Merton64, MertonDrv<Merton64>, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     default:
       return err.set(STOCHA_EINVAL, "unknown model id %d", model);
@@ -197,7 +346,8 @@ int run_greeks_pathwise(const GbmConst& gc, const StreamDesc& sd, uint64_t n_pat
   pay.sigma = sigma;
   pay.is_call = is_call;
   double h[2];
-  int rc = run_terminal<Gbm64, GbmConst, false>(gc, sd, n_paths, steps, pay, 2, h, st, err);
+  int rc = run_terminal<Gbm64, GbmDrv<Gbm64>, GbmConst, false>(gc, sd, n_paths, steps, pay, 2, h,
+                                                               kScheduleFeed, st, err);
   *delta_sum = h[0];
   *vega_sum = h[1];
   return rc;

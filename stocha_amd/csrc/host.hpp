@@ -148,9 +148,10 @@ int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n
                  uint64_t base_rows, uint64_t steps, double* d_out, hipStream_t st,
                  const Err& err);
 // fused.hip
-int launch_fused(int model, int precision, const void* mconst, const StreamDesc& sd,
-                 uint64_t n_paths, uint64_t steps, const double* strikes, int n_strikes,
-                 double* h_sums, hipStream_t st, const Err& err);
+enum Schedule : int { kScheduleFeed = 0, kScheduleLoop = 1 };
+int launch_fused(int model, int precision, int schedule, const void* mconst,
+                 const StreamDesc& sd, uint64_t n_paths, uint64_t steps, const double* strikes,
+                 int n_strikes, double* h_sums, hipStream_t st, const Err& err);
 // lsmc.hip
 int run_lsmc(const GbmConst& gc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
              double k, double discount, int is_put, int ncols, int precision, double* price,

@@ -16,6 +16,10 @@
 // They draw exactly the same normals/uniforms, in the same order, as the FP64
 // forms (see rng.hpp), so their error against the FP64 result is rounding
 // only.
+//
+// Each model separates the arithmetic of one step (apply*, given its normals)
+// from how the normals are drawn: step() draws with the per-normal ziggurat
+// loop; the state-machine drivers in fused.hip feed draws one at a time.
 #pragma once
 #include "rng.hpp"
 
@@ -47,75 +51,91 @@ __host__ __device__ inline uint64_t block_size(int model, uint64_t steps) {
 // ---------------------------------------------------------------- FP64 ----
 
 struct Gbm64 {
-  const GbmConst c;
+  static constexpr bool kFp32 = false;
+  GbmConst c;
   double s;
   bool neg;
   __device__ __forceinline__ Gbm64(const GbmConst& k, bool negate) : c(k), s(k.s0), neg(negate) {}
-  __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
-    double n = zig_normal64(g, inc, z);
+  __device__ __forceinline__ void apply(double n) {
     n = neg ? -n : n;
     s *= exp(c.drift + c.diffusion * n);
+  }
+  __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
+    apply(zig_normal64(g, inc, z));
     return s;
   }
+  __device__ __forceinline__ double terminal() const { return s; }
 };
 
 struct Heston64 {
-  const HestonConst c;
+  static constexpr bool kFp32 = false;
+  HestonConst c;
   double s, v;
   __device__ __forceinline__ Heston64(const HestonConst& k, bool) : c(k), s(k.s0), v(k.v0) {}
-  __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
-    double dw1 = zig_normal64(g, inc, z);
-    double dz = zig_normal64(g, inc, z);
+  __device__ __forceinline__ void apply(double dw1, double dz) {
     double dw2 = c.rho * dw1 + c.rho_comp * dz;
     double v_plus = fmax(v, 0.0);
     double sqrt_v = sqrt(v_plus);
     v += c.kappa * (c.theta - v_plus) * c.dt + c.xi * sqrt_v * c.sqrt_dt * dw2;
     s *= exp((c.mu - 0.5 * v_plus) * c.dt + sqrt_v * c.sqrt_dt * dw1);
+  }
+  __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
+    double dw1 = zig_normal64(g, inc, z);
+    double dz = zig_normal64(g, inc, z);
+    apply(dw1, dz);
     return s;
   }
+  __device__ __forceinline__ double terminal() const { return s; }
 };
 
 struct Merton64 {
-  const MertonConst c;
+  static constexpr bool kFp32 = false;
+  MertonConst c;
   double s;
   __device__ __forceinline__ Merton64(const MertonConst& k, bool) : c(k), s(k.s0) {}
+  __device__ __forceinline__ void apply(double zz, bool jump, double zj) {
+    double diffusion_return = c.drift + c.diffusion * zz;
+    double jump_return = 0.0;
+    if (jump) jump_return = c.mu_j + c.sigma_j * zj;
+    s *= exp(diffusion_return + jump_return);
+  }
   __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
     double zz = zig_normal64(g, inc, z);
-    double diffusion_return = c.drift + c.diffusion * zz;
     double u = g.next_f64(inc);
-    double jump_return = 0.0;
-    if (u < c.jump_prob) {
-      double zj = zig_normal64(g, inc, z);
-      jump_return = c.mu_j + c.sigma_j * zj;
-    }
-    s *= exp(diffusion_return + jump_return);
+    bool jump = u < c.jump_prob;
+    double zj = jump ? zig_normal64(g, inc, z) : 0.0;
+    apply(zz, jump, zj);
     return s;
   }
+  __device__ __forceinline__ double terminal() const { return s; }
 };
 
 // ---------------------------------------------------------------- FP32 ----
 // step() returns the log-price increment's running sum; s0 * exp(ls) is S.
 
 struct Gbm32 {
-  const GbmConst c;
+  static constexpr bool kFp32 = true;
+  GbmConst c;
   float ls;
   bool neg;
   __device__ __forceinline__ Gbm32(const GbmConst& k, bool negate) : c(k), ls(0.f), neg(negate) {}
-  __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
-    float n = zig_normal32(g, inc, z);
+  __device__ __forceinline__ void apply(float n) {
     n = neg ? -n : n;
     ls += fmaf(c.diffusion_f, n, c.drift_f);
+  }
+  __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
+    apply(zig_normal32(g, inc, z));
     return ls;
   }
+  __device__ __forceinline__ double terminal() const { return c.s0 * exp((double)ls); }
 };
 
 struct Heston32 {
-  const HestonConst c;
+  static constexpr bool kFp32 = true;
+  HestonConst c;
   float ls, v;
   __device__ __forceinline__ Heston32(const HestonConst& k, bool) : c(k), ls(0.f), v(k.v0_f) {}
-  __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
-    float dw1 = zig_normal32(g, inc, z);
-    float dz = zig_normal32(g, inc, z);
+  __device__ __forceinline__ void apply(float dw1, float dz) {
     float dw2 = fmaf(c.rho_f, dw1, c.rho_comp_f * dz);
     float v_plus = fmaxf(v, 0.f);
     float sqrt_v = __builtin_sqrtf(v_plus);
@@ -123,25 +143,35 @@ struct Heston32 {
     v = fmaf(c.kappa_dt_f, c.theta_f - v_plus, fmaf(c.xi_sqdt_f * sqrt_v, dw2, v));
     // log S += (mu - v+/2)*dt + sqrt(v+)*sqrt(dt)*dw1
     ls += fmaf(-c.half_dt_f, v_plus, fmaf(sqrt_v * c.sqrt_dt_f, dw1, c.mu_dt_f));
+  }
+  __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
+    float dw1 = zig_normal32(g, inc, z);
+    float dz = zig_normal32(g, inc, z);
+    apply(dw1, dz);
     return ls;
   }
+  __device__ __forceinline__ double terminal() const { return c.s0 * exp((double)ls); }
 };
 
 struct Merton32 {
-  const MertonConst c;
+  static constexpr bool kFp32 = true;
+  MertonConst c;
   float ls;
   __device__ __forceinline__ Merton32(const MertonConst& k, bool) : c(k), ls(0.f) {}
+  __device__ __forceinline__ void apply(float zz, bool jump, float zj) {
+    float r = fmaf(c.diffusion_f, zz, c.drift_f);
+    if (jump) r += fmaf(c.sigma_j_f, zj, c.mu_j_f);
+    ls += r;
+  }
   __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
     float zz = zig_normal32(g, inc, z);
-    float r = fmaf(c.diffusion_f, zz, c.drift_f);
     double u = g.next_f64(inc);  // exact, same comparison as FP64
-    if (u < c.jump_prob) {
-      float zj = zig_normal32(g, inc, z);
-      r += fmaf(c.sigma_j_f, zj, c.mu_j_f);
-    }
-    ls += r;
+    bool jump = u < c.jump_prob;
+    float zj = jump ? zig_normal32(g, inc, z) : 0.f;
+    apply(zz, jump, zj);
     return ls;
   }
+  __device__ __forceinline__ double terminal() const { return c.s0 * exp((double)ls); }
 };
 
 }  // namespace stocha

@@ -134,15 +134,27 @@ __device__ __forceinline__ void zig_load(ZigLds* z) {
   }
 }
 
-// normal.rs:54-67 (always FP64: both forms consume the same draws)
-__device__ __noinline__ double zig_tail(Pcg& g, const U128 inc) {
+// normal.rs:54-67 (always FP64: both forms consume the same draws).
+// Kept out of line (it runs for ~0.02% of attempts); the generator state goes
+// in and comes back by value so the caller's state stays in registers.
+struct TailOut {
+  double x;
+  U128 s;
+};
+__device__ __noinline__ TailOut zig_tail_impl(U128 s, const U128 inc) {
+  Pcg g{s};
   for (;;) {
     double u1 = g.next_f64(inc);
     double u2 = g.next_f64(inc);
     if (u1 <= 0.0 || u2 <= 0.0) continue;
     double x = -log(u1) / kZigR;
-    if (-2.0 * log(u2) >= x * x) return x + kZigR;
+    if (-2.0 * log(u2) >= x * x) return TailOut{x + kZigR, g.s};
   }
+}
+__device__ __forceinline__ double zig_tail(Pcg& g, const U128 inc) {
+  TailOut t = zig_tail_impl(g.s, inc);
+  g.s = t.s;
+  return t.x;
 }
 
 // normal.rs:18-44, FP64 (bit-exact with the reference up to the libm exp/log

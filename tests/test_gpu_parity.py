@@ -183,3 +183,26 @@ def test_greeks_pathwise_matches_oracle(gpu, orc, is_call):
     assert list(g) == ["delta", "vega"]
     assert g["delta"] == pytest.approx(d, rel=RTOL64)
     assert g["vega"] == pytest.approx(v, rel=RTOL64)
+
+
+@pytest.mark.parametrize("model,params", [("gbm", GBM), ("heston", HESTON),
+                                          ("merton", MERTON)])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_feed_schedule_equals_loop(gpu, model, params, prec):
+    # the per-lane state machine consumes each path's draws in the same order
+    # as the per-normal loop, so both schedules give the same bits
+    kw = dict(t=1.0, steps=100, n_paths=3000, strikes=STRIKES, r=0.05, seed=7, precision=prec)
+    a = gpu.european_price(model, params, schedule="feed", **kw)
+    b = gpu.european_price(model, params, schedule="loop", **kw)
+    np.testing.assert_array_equal(a["sum"], b["sum"])
+    np.testing.assert_array_equal(a["sumsq"], b["sumsq"])
+
+
+def test_feed_multiple_paths_per_thread(gpu, orc):
+    # more paths than one grid holds: lanes start new paths inside the loop
+    n, steps = 1_200_000, 4
+    res = gpu.european_price("heston", HESTON, 1.0, steps, n, STRIKES, r=0.05, seed=3,
+                             precision=64)
+    term = orc.terminals(1, list(HESTON.values()), 1.0, steps, n, 3)
+    s1, _ = orc.call_sums(term, STRIKES)
+    np.testing.assert_allclose(res["sum"], s1, rtol=RTOL64)
