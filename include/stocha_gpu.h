@@ -160,6 +160,52 @@ int stocha_lsmc_ex(double s0, double k, double r, double sigma, double t, uint64
                    uint64_t n_paths, int is_put, uint64_t poly_degree, uint64_t seed,
                    int precision, double *price, double *std_err, char *err, size_t errlen);
 
+/* Weighted basket call max(sum_a w_a S_T,a - K, 0) on correlated GBM with the
+ * streams and draw order of multi_gbm (multi_gbm.rs:40-64; d <= 64), priced
+ * in-register, and -- when greeks != 0 -- the pathwise adjoint of the price
+ * with respect to each asset's s0, mu and sigma (d_s0/d_mu/d_sigma, length d,
+ * host).  corr is d x d (Cholesky as copula/gaussian.rs:76-99, same error).
+ * price = exp(-r t) mean payoff.  precision STOCHA_FP32 sums the normals and
+ * forms L*W in FP32, STOCHA_FP64 in FP64.  No reference counterpart. */
+int stocha_basket(const double *s0, const double *mu, const double *sigma, const double *corr,
+                  const double *weights, uint64_t d, double strike, double r, double t,
+                  uint64_t steps, uint64_t n_paths, uint64_t seed, int precision, int greeks,
+                  double *price, double *std_err, double *d_s0, double *d_mu, double *d_sigma,
+                  char *err, size_t errlen);
+
+/* ---- batched pricing: many uneven instances in one persistent launch ------
+ * Instance i prices a European call at `strike` on `n_paths` paths of
+ * `steps` steps of its model, on the reference's block-split streams of the
+ * plan's seed (the paths the single-instance call with that seed would
+ * draw), FP32 log-space stepping, FP64 sums; price = exp(-r t) * mean.
+ * Fields used per model: GBM s0, mu, sigma; Heston s0, v0, mu, kappa, theta,
+ * xi, rho; Merton s0, mu, sigma, lambda_, mu_j, sigma_j.  No reference
+ * counterpart (the reference prices one instance per call). */
+typedef struct {
+  int model; /* STOCHA_MODEL_* */
+  uint32_t steps;
+  uint64_t n_paths;
+  double s0, strike, r, t;
+  double mu, sigma;
+  double v0, kappa, theta, xi, rho;
+  double lambda_, mu_j, sigma_j;
+} stocha_batch_instance;
+
+typedef struct stocha_batch_plan stocha_batch_plan;
+
+/* Build a plan: device tables, cost-ordered path tiles of block *
+ * paths_per_lane paths, a persistent grid (waves_per_simd <= 0: as many
+ * blocks as are resident), all captured in a hipGraph.  block: 64/128/256. */
+int stocha_batch_plan_create(const stocha_batch_instance *inst, uint64_t n, uint64_t seed,
+                             int block, int paths_per_lane, int waves_per_simd,
+                             stocha_batch_plan **plan, char *err, size_t errlen);
+/* Replay the plan's graph; price/std_err have n entries (host). */
+int stocha_batch_plan_run(stocha_batch_plan *plan, double *price, double *std_err, char *err,
+                          size_t errlen);
+/* Tiles and persistent grid size of a plan (for tuning reports). */
+int stocha_batch_plan_info(const stocha_batch_plan *plan, uint64_t *n_tiles, uint64_t *grid);
+void stocha_batch_plan_destroy(stocha_batch_plan *plan);
+
 /* ---- random-number primitives (pcg64.rs:23-43, normal.rs:18-74) ---------
  * Draws of the block-split streams of paths path0 .. path0+n_paths-1, where
  * path i starts at Pcg64Dxsm::new(seed) advanced by i*block (the steppers'

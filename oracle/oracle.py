@@ -48,6 +48,8 @@ def lib():
         L.orc_multi_gbm.argtypes = [_u, _pd, _pd, _pd, _pd, _d, _u, _u, _u, _i, _pd,
                                     C.c_char_p, C.c_size_t]
         L.orc_cholesky.argtypes = [_u, _pd, _pd, C.c_char_p, C.c_size_t]
+        L.orc_multi_terminals.argtypes = [_u, _pd, _pd, _pd, _pd, _d, _u, _u, _u, _pd,
+                                          C.c_char_p, C.c_size_t]
         L.orc_terminals.argtypes = [_i, _pd, _d, _u, _u, _u, _pd]
         L.orc_call_sums.argtypes = [_pd, _u, _pd, _i, _pd, _pd]
         L.orc_call_sums.restype = None
@@ -140,6 +142,21 @@ def multi_gbm(s0, mu, sigma, corr, t, steps, n_paths, seed=42, antithetic=False)
     err = C.create_string_buffer(256)
     rc = lib().orc_multi_gbm(d, _p(s0), _p(mu), _p(sigma), _p(corr), t, steps, n_paths, seed,
                              int(antithetic), _p(out), err, 256)
+    if rc:
+        raise ValueError(err.value.decode())
+    return out
+
+
+def multi_terminals(s0, mu, sigma, corr, t, steps, n_paths, seed=42):
+    d = len(s0)
+    s0 = np.ascontiguousarray(s0, np.float64)
+    mu = np.ascontiguousarray(mu, np.float64)
+    sigma = np.ascontiguousarray(sigma, np.float64)
+    corr = np.ascontiguousarray(corr, np.float64)
+    out = np.empty((n_paths, d), np.float64)
+    err = C.create_string_buffer(256)
+    rc = lib().orc_multi_terminals(d, _p(s0), _p(mu), _p(sigma), _p(corr), t, steps, n_paths,
+                                   seed, _p(out), err, 256)
     if rc:
         raise ValueError(err.value.decode())
     return out

@@ -393,9 +393,12 @@ int orc_cholesky(uint64_t n, const double *a, double *l, char *err, size_t errle
 /* ------------------------------------------------------------------------ */
 /* Multi-asset GBM: multi_gbm.rs:22-113                                      */
 /* ------------------------------------------------------------------------ */
-int orc_multi_gbm(uint64_t d, const double *s0, const double *mu, const double *sigma,
-                  const double *corr, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
-                  int antithetic, double *out, char *err, size_t errlen) {
+/* multi_gbm.rs:22-113.  Stores the full path (out != NULL) or only the
+ * terminal row (term != NULL). */
+static int multi_core(uint64_t d, const double *s0, const double *mu, const double *sigma,
+                      const double *corr, double t, uint64_t steps, uint64_t n_paths,
+                      uint64_t seed, int antithetic, double *out, double *term, char *err,
+                      size_t errlen) {
     double *L = (double *)malloc(sizeof(double) * d * d);
     double *drift = (double *)malloc(sizeof(double) * d);
     double *diff = (double *)malloc(sizeof(double) * d);
@@ -414,8 +417,8 @@ int orc_multi_gbm(uint64_t d, const double *s0, const double *mu, const double *
     double dt = t / (double)steps;
     double sqrt_dt = sqrt(dt);
     for (uint64_t i = 0; i < d; i++) {
-        drift[i] = (mu[i] - 0.5 * sigma[i] * sigma[i]) * dt;
-        diff[i] = sigma[i] * sqrt_dt;
+        drift[i] = (mu[i] - 0.5 * sigma[i] * sigma[i]) * dt; /* :29-31 */
+        diff[i] = sigma[i] * sqrt_dt;                          /* :32 */
     }
     uint64_t base = antithetic ? (n_paths + 1) / 2 : n_paths;
     uint64_t block = (steps * d + 1024) * 2; /* multi_gbm.rs:40 */
@@ -436,22 +439,24 @@ int orc_multi_gbm(uint64_t d, const double *s0, const double *mu, const double *
                 uint64_t path = negate ? row - base : row;
                 pcg_t g;
                 pcg_for_path(&g, seed, block, path);
-                double *o = out + row * row_len;
+                double *o = out ? out + row * row_len : NULL;
                 for (uint64_t i = 0; i < d; i++) {
                     s[i] = s0[i];
-                    o[i] = s0[i];
+                    if (o) o[i] = s0[i];
                 }
                 for (uint64_t j = 0; j < steps; j++) {
-                    for (uint64_t i = 0; i < d; i++) z[i] = zig_normal(&g);
+                    for (uint64_t i = 0; i < d; i++) z[i] = zig_normal(&g); /* :59 */
                     if (negate)
-                        for (uint64_t i = 0; i < d; i++) z[i] = -z[i];
-                    for (uint64_t i = 0; i < d; i++) {
+                        for (uint64_t i = 0; i < d; i++) z[i] = -z[i]; /* :60-64 */
+                    for (uint64_t i = 0; i < d; i++) {                /* :67-74 */
                         double eps = 0.0;
                         for (uint64_t k = 0; k <= i; k++) eps += L[i * d + k] * z[k];
                         s[i] *= exp(drift[i] + diff[i] * eps);
-                        o[(j + 1) * d + i] = s[i];
+                        if (o) o[(j + 1) * d + i] = s[i];
                     }
                 }
+                if (term)
+                    for (uint64_t i = 0; i < d; i++) term[row * d + i] = s[i];
             }
         }
         free(z);
@@ -461,6 +466,19 @@ int orc_multi_gbm(uint64_t d, const double *s0, const double *mu, const double *
     free(drift);
     free(diff);
     return rc;
+}
+
+int orc_multi_gbm(uint64_t d, const double *s0, const double *mu, const double *sigma,
+                  const double *corr, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
+                  int antithetic, double *out, char *err, size_t errlen) {
+    return multi_core(d, s0, mu, sigma, corr, t, steps, n_paths, seed, antithetic, out, NULL, err,
+                      errlen);
+}
+
+int orc_multi_terminals(uint64_t d, const double *s0, const double *mu, const double *sigma,
+                        const double *corr, double t, uint64_t steps, uint64_t n_paths,
+                        uint64_t seed, double *out, char *err, size_t errlen) {
+    return multi_core(d, s0, mu, sigma, corr, t, steps, n_paths, seed, 0, NULL, out, err, errlen);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -47,6 +47,10 @@ int orc_multi_gbm(uint64_t d, const double *s0, const double *mu, const double *
                   const double *corr, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
                   int antithetic, double *out, char *err, size_t errlen);
 int orc_cholesky(uint64_t n, const double *a, double *l, char *err, size_t errlen);
+/* Terminal prices only, same arithmetic as orc_multi_gbm: out is (n_paths, d). */
+int orc_multi_terminals(uint64_t d, const double *s0, const double *mu, const double *sigma,
+                        const double *corr, double t, uint64_t steps, uint64_t n_paths,
+                        uint64_t seed, double *out, char *err, size_t errlen);
 
 /* ---- terminal-only steppers (same arithmetic as above, last column only) ---- */
 /* model: 0 GBM (p = s0, mu, sigma, q), 1 Heston (s0, v0, mu, kappa, theta, xi, rho),

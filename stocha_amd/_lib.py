@@ -55,6 +55,12 @@ _SIGS = {
     "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _pd, _pd, _pd,
                                _ps, _sz], C.c_int),
     "stocha_lsmc_ex": ([_d] * 5 + [_u, _u, _i, _u, _u, _i, _pd, _pd, _ps, _sz], C.c_int),
+    "stocha_basket": ([_pd, _pd, _pd, _pd, _pd, _u, _d, _d, _d, _u, _u, _u, _i, _i, _pd, _pd,
+                       _pd, _pd, _pd, _ps, _sz], C.c_int),
+    "stocha_batch_plan_create": ([_vp, _u, _u, _i, _i, _i, C.POINTER(_vp), _ps, _sz], C.c_int),
+    "stocha_batch_plan_run": ([_vp, _pd, _pd, _ps, _sz], C.c_int),
+    "stocha_batch_plan_info": ([_vp, _pu, _pu], C.c_int),
+    "stocha_batch_plan_destroy": ([_vp], None),
     "stocha_stream_u64": ([_u, _u, _u, _u, _u, _pu, _ps, _sz], C.c_int),
     "stocha_stream_normals": ([_u, _u, _u, _u, _u, _i, _pd, _pu, _ps, _sz], C.c_int),
 }

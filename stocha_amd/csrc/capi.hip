@@ -7,6 +7,7 @@
 // multi_gbm.rs:24-40, lsmc.rs:26-27), sets up the block-split stream table
 // and launches the kernels on the work stream.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -424,6 +425,109 @@ int stocha_european_fused(int model, const double* params, uint64_t n_params, do
     }
   }
   return STOCHA_OK;
+}
+
+// Basket call on correlated GBM (multi_gbm.rs streams) with pathwise adjoint
+// Greeks.  No reference counterpart.
+int stocha_basket(const double* s0, const double* mu, const double* sigma, const double* corr,
+                  const double* weights, uint64_t d, double strike, double r, double t,
+                  uint64_t steps, uint64_t n_paths, uint64_t seed, int precision, int greeks,
+                  double* price, double* std_err, double* d_s0, double* d_mu, double* d_sigma,
+                  char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  if (d == 0) return err.set(STOCHA_EINVAL, "s0 must not be empty");
+  if (d > 64) return err.set(STOCHA_EINVAL, "the basket form supports up to 64 assets, got %llu",
+                             (unsigned long long)d);
+  for (uint64_t i = 0; i < d; ++i)
+    if (s0[i] <= 0.0) return err.set(STOCHA_EINVAL, "all s0 must be positive");
+  for (uint64_t i = 0; i < d; ++i)
+    if (sigma[i] <= 0.0) return err.set(STOCHA_EINVAL, "all sigma must be positive");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  if (steps * d > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps * d must be below 2^32");
+  if (precision != STOCHA_FP64 && precision != STOCHA_FP32)
+    return err.set(STOCHA_EINVAL, "precision must be 32 or 64");
+  std::vector<double> L(d * d);
+  if (int rc = cholesky_host(corr, d, L.data(), err)) return rc;
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  std::vector<double> h(2 + 3 * d);
+  // FP32 forms L*W on the matrix cores unless STOCHA_BASKET_VALU is set
+  const bool mfma = precision == STOCHA_FP32 && !getenv("STOCHA_BASKET_VALU");
+  if (int rc = run_basket(s0, mu, sigma, L.data(), weights, d, strike, t, steps, n_paths, seed,
+                          precision, greeks != 0, mfma, h.data(), globals().stream, err))
+    return rc;
+  const double disc = exp(-r * t);
+  const double n = (double)n_paths;
+  const double s1 = h[0], s2 = h[1];
+  *price = disc * s1 / n;
+  double var = n > 1 ? (s2 - s1 * s1 / n) / (n - 1.0) : 0.0;
+  var = var > 0.0 ? var : 0.0;
+  if (std_err) *std_err = disc * sqrt(var / n);
+  if (greeks) {
+    for (uint64_t a = 0; a < d; ++a) {
+      if (d_s0) d_s0[a] = disc * h[2 + 3 * a] / n;
+      if (d_mu) d_mu[a] = disc * h[3 + 3 * a] / n;
+      if (d_sigma) d_sigma[a] = disc * h[4 + 3 * a] / n;
+    }
+  }
+  return STOCHA_OK;
+}
+
+struct stocha_batch_plan {
+  stocha::BatchPlan* p;
+};
+
+int stocha_batch_plan_create(const stocha_batch_instance* inst, uint64_t n, uint64_t seed,
+                             int block, int paths_per_lane, int waves_per_simd,
+                             stocha_batch_plan** plan, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  if (n == 0 || n > 0xffffffffull) return err.set(STOCHA_EINVAL, "n must be in [1, 2^32)");
+  for (uint64_t i = 0; i < n; ++i) {
+    const stocha_batch_instance& b = inst[i];
+    if (b.model < 0 || b.model > 2)
+      return err.set(STOCHA_EINVAL, "instance %llu: unknown model %d", (unsigned long long)i,
+                     b.model);
+    if (b.steps == 0 || b.n_paths == 0)
+      return err.set(STOCHA_EINVAL, "instance %llu: steps and n_paths must be positive",
+                     (unsigned long long)i);
+    if (b.s0 <= 0.0 || b.t <= 0.0)
+      return err.set(STOCHA_EINVAL, "instance %llu: s0 and t must be positive",
+                     (unsigned long long)i);
+    if (b.model != STOCHA_MODEL_HESTON && b.sigma <= 0.0)
+      return err.set(STOCHA_EINVAL, "instance %llu: sigma must be positive",
+                     (unsigned long long)i);
+  }
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  stocha::BatchPlan* p = nullptr;
+  if (int rc = batch_plan_create(inst, n, seed, block, paths_per_lane, waves_per_simd, &p, err))
+    return rc;
+  *plan = new stocha_batch_plan{p};
+  return STOCHA_OK;
+}
+
+int stocha_batch_plan_run(stocha_batch_plan* plan, double* price, double* std_err, char* errbuf,
+                          size_t errlen) {
+  Err err{errbuf, errlen};
+  if (!plan) return err.set(STOCHA_EINVAL, "null plan");
+  std::lock_guard<std::mutex> lk(globals().mu);
+  return batch_plan_run(plan->p, price, std_err, err);
+}
+
+int stocha_batch_plan_info(const stocha_batch_plan* plan, uint64_t* n_tiles, uint64_t* grid) {
+  if (!plan) return STOCHA_EINVAL;
+  if (n_tiles) *n_tiles = batch_plan_tiles(plan->p);
+  if (grid) *grid = batch_plan_grid(plan->p);
+  return STOCHA_OK;
+}
+
+void stocha_batch_plan_destroy(stocha_batch_plan* plan) {
+  if (!plan) return;
+  std::lock_guard<std::mutex> lk(globals().mu);
+  batch_plan_destroy(plan->p);
+  delete plan;
 }
 
 }  // extern "C"

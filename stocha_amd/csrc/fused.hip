@@ -16,7 +16,7 @@
 //                  accumulates its payoff and starts its next path inside the
 //                  same loop, so lanes never wait on each other.
 #include "host.hpp"
-#include "zigsm.hpp"
+#include "drivers.hpp"
 
 namespace stocha {
 
@@ -72,93 +72,6 @@ __device__ __forceinline__ double terminal_value(const Const& mc, const StreamDe
     return m.s;
   }
 }
-
-// ------------------------------------------------ state-machine drivers --
-// feed(u) consumes one draw; returns true when the path's last step is done.
-template <class M>
-struct GbmDrv {
-  using Real = typename ZigFeed<M::kFp32>::Real;
-  M m;
-  ZigFeed<M::kFp32> zf;
-  uint32_t done = 0;
-  template <class C>
-  __device__ __forceinline__ explicit GbmDrv(const C& c) : m(c, false) {}
-  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
-                                       uint32_t steps) {
-    Real x;
-    if (zf.feed(u, g, inc, z, x)) {
-      m.apply(x);
-      return ++done == steps;
-    }
-    return false;
-  }
-};
-
-template <class M>
-struct HestonDrv {
-  using Real = typename ZigFeed<M::kFp32>::Real;
-  M m;
-  ZigFeed<M::kFp32> zf;
-  uint32_t done = 0;
-  bool have = false;
-  Real dw1 = 0;
-  template <class C>
-  __device__ __forceinline__ explicit HestonDrv(const C& c) : m(c, false) {}
-  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
-                                       uint32_t steps) {
-    Real x;
-    if (zf.feed(u, g, inc, z, x)) {
-      if (!have) {
-        dw1 = x;
-        have = true;
-        return false;
-      }
-      have = false;
-      m.apply(dw1, x);
-      return ++done == steps;
-    }
-    return false;
-  }
-};
-
-// Merton's per-step grammar: normal z, then one uniform (always), then a
-// jump normal when the uniform is below lambda*dt (jump_diffusion.rs:75-85).
-template <class M>
-struct MertonDrv {
-  using Real = typename ZigFeed<M::kFp32>::Real;
-  M m;
-  ZigFeed<M::kFp32> zf;
-  uint32_t done = 0;
-  int phase = 0;  // 0: drawing z, 1: next draw is the jump uniform, 2: drawing z_j
-  Real zz = 0;
-  template <class C>
-  __device__ __forceinline__ explicit MertonDrv(const C& c) : m(c, false) {}
-  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
-                                       uint32_t steps) {
-    if (phase == 1) {
-      const double uf = (double)(u >> 11) * 0x1p-53;
-      if (uf < m.c.jump_prob) {
-        phase = 2;
-        return false;
-      }
-      phase = 0;
-      m.apply(zz, false, (Real)0);
-      return ++done == steps;
-    }
-    Real x;
-    if (zf.feed(u, g, inc, z, x)) {
-      if (phase == 0) {
-        zz = x;
-        phase = 1;
-        return false;
-      }
-      phase = 0;
-      m.apply(zz, true, x);
-      return ++done == steps;
-    }
-    return false;
-  }
-};
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -269,6 +182,13 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __re
     if (threadIdx.x == 0) out[i] = buf[0];
     __syncthreads();
   }
+}
+
+int launch_reduce_partials(const double* part, int n_blocks, int na, double* out, hipStream_t st,
+                           const Err& err) {
+  reduce_partials_kernel<<<1, 256, 0, st>>>(part, n_blocks, na, out);
+  STOCHA_HIP(hipGetLastError());
+  return STOCHA_OK;
 }
 
 static unsigned fused_grid(uint64_t n_paths) {

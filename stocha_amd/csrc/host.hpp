@@ -7,6 +7,7 @@
 #include <stdio.h>
 
 #include <mutex>
+#include <vector>
 
 #include "models.hpp"
 #include "stocha_gpu.h"
@@ -152,6 +153,22 @@ enum Schedule : int { kScheduleFeed = 0, kScheduleLoop = 1 };
 int launch_fused(int model, int precision, int schedule, const void* mconst,
                  const StreamDesc& sd, uint64_t n_paths, uint64_t steps, const double* strikes,
                  int n_strikes, double* h_sums, hipStream_t st, const Err& err);
+// Fixed-order sum of partials[b][i] over b into out[i] (one block, on st).
+int launch_reduce_partials(const double* part, int n_blocks, int na, double* out, hipStream_t st,
+                           const Err& err);
+// basket.hip: h_sums = {sum payoff, sum payoff^2, d x (dP/ds0, dP/dmu, dP/dsigma)}
+int run_basket(const double* s0, const double* mu, const double* sigma, const double* L,
+               const double* w, uint64_t d, double strike, double t, uint64_t steps,
+               uint64_t n_paths, uint64_t seed, int precision, bool greeks, bool mfma,
+               double* h_sums, hipStream_t st, const Err& err);
+// batch.hip
+struct BatchPlan;
+int batch_plan_create(const stocha_batch_instance* in, uint64_t n, uint64_t seed, int block,
+                      int paths_per_lane, int waves_per_simd, BatchPlan** out, const Err& err);
+int batch_plan_run(BatchPlan* P, double* price, double* std_err, const Err& err);
+void batch_plan_destroy(BatchPlan* P);
+uint32_t batch_plan_tiles(const BatchPlan* P);
+unsigned batch_plan_grid(const BatchPlan* P);
 // lsmc.hip
 int run_lsmc(const GbmConst& gc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
              double k, double discount, int is_put, int ncols, int precision, double* price,

@@ -1,0 +1,129 @@
+"""Multi-asset basket call with pathwise adjoint Greeks (GPU) against the oracle.
+
+The oracle simulates multi_gbm's paths with the reference's per-step
+arithmetic (multi_gbm.rs:67-74); the basket payoff and the pathwise Greeks
+are formed here in NumPy from its terminal prices, using the log-return form
+of greeks.rs:403-418 per asset:
+    vega_a = e^{-rT}/n sum_ITM w_a S_a (ln(S_a/s0_a) - (mu_a + sigma_a^2/2) T) / sigma_a.
+The GPU forms L*(sum_t z_t) once per path (reassociation of the reference's
+per-step L z_t; ~1e-14 relative), so the FP64 bar here is 1e-11.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def config5_inputs(d=64, seed=5):
+    """Config-5 style inputs: C = A A^T / d + 0.1 I rescaled to unit diagonal."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((d, d))
+    c = a @ a.T / d + 0.1 * np.eye(d)
+    dg = np.sqrt(np.diag(c))
+    corr = c / np.outer(dg, dg)
+    s0 = rng.uniform(50, 200, d)
+    mu = rng.uniform(0.01, 0.08, d)
+    sigma = rng.uniform(0.1, 0.4, d)
+    w = np.full(d, 1.0 / d)
+    return s0, mu, sigma, corr, w, float(w @ s0)
+
+
+def oracle_basket(orc, s0, mu, sigma, corr, w, k, r, t, steps, n, seed):
+    term = orc.multi_terminals(s0, mu, sigma, corr, t, steps, n, seed)
+    b = term @ w
+    pay = np.maximum(b - k, 0.0)
+    disc = math.exp(-r * t)
+    itm = b > k
+    sb = term[itm] * w  # Sbar_a * S_a per ITM path
+    delta = disc * (sb / s0).sum(0) / n
+    dmu = disc * (sb * t).sum(0) / n
+    lr = np.log(term[itm] / s0)
+    vega = disc * (sb * (lr - (mu + 0.5 * sigma**2) * t) / sigma).sum(0) / n
+    return disc * pay.mean(), delta, dmu, vega
+
+
+@pytest.mark.parametrize("d", [1, 3, 8])
+def test_basket_fp64_matches_oracle(gpu, orc, d):
+    s0, mu, sigma, corr, w, k = config5_inputs(d, seed=d)
+    kw = dict(r=0.03, t=1.0, steps=40, n_paths=3000, seed=11)
+    res = gpu.basket_price(s0, mu, sigma, corr, w, k, precision=64, greeks=True, **kw)
+    p, dl, dm, vg = oracle_basket(orc, s0, mu, sigma, corr, w, k, kw["r"], kw["t"], kw["steps"],
+                                  kw["n_paths"], kw["seed"])
+    assert res["price"] == pytest.approx(p, rel=1e-11)
+    np.testing.assert_allclose(res["delta"], dl, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(res["dmu"], dm, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(res["vega"], vg, rtol=1e-8, atol=1e-10)
+
+
+def test_basket_fp32_matches_oracle(gpu, orc):
+    s0, mu, sigma, corr, w, k = config5_inputs(64)
+    kw = dict(r=0.03, t=1.0, steps=20, n_paths=4096, seed=3)
+    res = gpu.basket_price(s0, mu, sigma, corr, w, k, precision=32, greeks=True, **kw)
+    p, dl, dm, vg = oracle_basket(orc, s0, mu, sigma, corr, w, k, kw["r"], kw["t"], kw["steps"],
+                                  kw["n_paths"], kw["seed"])
+    assert abs(res["price"] - p) / p < 1e-4
+    np.testing.assert_allclose(res["delta"], dl, rtol=1e-3, atol=1e-7)
+    np.testing.assert_allclose(res["vega"], vg, rtol=1e-3, atol=1e-4)
+
+
+def test_basket_d1_equals_greeks_pathwise(gpu):
+    # d = 1, w = 1, mu = r: the basket adjoint is greeks.rs's pathwise delta/vega
+    kw = dict(t=1.0, steps=252, n_paths=20_000, seed=42)
+    g = gpu.greeks_pathwise(s0=100.0, r=0.05, sigma=0.2, strike=100.0, is_call=True,
+                            n_paths=kw["n_paths"], n_steps=kw["steps"], greeks=["delta", "vega"],
+                            t=kw["t"], seed=kw["seed"])
+    b = gpu.basket_price([100.0], [0.05], [0.2], np.eye(1), [1.0], 100.0, r=0.05, precision=64,
+                         greeks=True, **kw)
+    assert b["delta"][0] == pytest.approx(g["delta"], rel=1e-11)
+    assert b["vega"][0] == pytest.approx(g["vega"], rel=1e-9)
+
+
+def test_basket_adjoint_vs_crn_finite_differences(gpu):
+    s0, mu, sigma, corr, w, k = config5_inputs(8, seed=2)
+    kw = dict(r=0.03, t=1.0, steps=30, n_paths=20_000, seed=9, precision=64)
+    res = gpu.basket_price(s0, mu, sigma, corr, w, k, greeks=True, **kw)
+    for a in (0, 5):
+        for name, arr, key in (("s0", s0, "delta"), ("sigma", sigma, "vega"),
+                               ("mu", mu, "dmu")):
+            h = 1e-4 * arr[a]
+            up, dn = arr.copy(), arr.copy()
+            up[a] += h
+            dn[a] -= h
+            args = {"s0": s0, "mu": mu, "sigma": sigma}
+            pu = gpu.basket_price(**{**args, name: up}, corr=corr, weights=w, strike=k, **kw)
+            pd = gpu.basket_price(**{**args, name: dn}, corr=corr, weights=w, strike=k, **kw)
+            fd = (pu["price"] - pd["price"]) / (2 * h)
+            assert res[key][a] == pytest.approx(fd, rel=2e-3, abs=1e-5), (name, a)
+
+
+def test_autograd_backward_replays_adjoint(gpu):
+    import torch
+    from stocha_amd.autograd import basket_call
+    s0, mu, sigma, corr, w, k = config5_inputs(16, seed=4)
+    ts0 = torch.tensor(s0, dtype=torch.float64, requires_grad=True)
+    tmu = torch.tensor(mu, dtype=torch.float64, requires_grad=True)
+    tsg = torch.tensor(sigma, dtype=torch.float64, requires_grad=True)
+    price = basket_call(ts0, tmu, tsg, corr, w, k, 0.03, 1.0, 25, 8192, seed=1, precision=64)
+    (2.0 * price).backward()
+    ref = gpu.basket_price(s0, mu, sigma, corr, w, k, 0.03, 1.0, 25, 8192, seed=1, precision=64,
+                           greeks=True)
+    assert price.item() == pytest.approx(ref["price"], rel=1e-15)
+    np.testing.assert_allclose(ts0.grad.numpy(), 2 * ref["delta"], rtol=1e-14)
+    np.testing.assert_allclose(tmu.grad.numpy(), 2 * ref["dmu"], rtol=1e-14)
+    np.testing.assert_allclose(tsg.grad.numpy(), 2 * ref["vega"], rtol=1e-14)
+
+
+@pytest.mark.parametrize("d", [64, 20, 5])
+def test_basket_mfma_epilogue_equals_valu(gpu, monkeypatch, d):
+    # the FP32 form forms L*W on the matrix cores; the VALU epilogue is the check
+    s0, mu, sigma, corr, w, k = config5_inputs(d, seed=10 + d)
+    kw = dict(r=0.03, t=1.0, steps=12, n_paths=3000, seed=2, precision=32, greeks=True)
+    m = gpu.basket_price(s0, mu, sigma, corr, w, k, **kw)
+    monkeypatch.setenv("STOCHA_BASKET_VALU", "1")
+    v = gpu.basket_price(s0, mu, sigma, corr, w, k, **kw)
+    assert m["price"] == pytest.approx(v["price"], rel=2e-6)
+    np.testing.assert_allclose(m["delta"], v["delta"], rtol=2e-5, atol=1e-9)
+    np.testing.assert_allclose(m["vega"], v["vega"], rtol=2e-5, atol=1e-7)
+    np.testing.assert_allclose(m["dmu"], v["dmu"], rtol=2e-5, atol=1e-7)
