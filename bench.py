@@ -37,16 +37,24 @@ R = 0.05
 T = 1.0
 SEED = 42
 
-# MI355X published peaks
-PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12  # FP64 / non-packed FP32 lane-ops (39.3 T/s)
+# MI355X peaks: HBM 8 TB/s; VALU issue 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
+# = 39.3 T lane-cycles/s (FP32 and FP64 FMA are full rate on CDNA4; 32-bit
+# integer multiplies, v_mad_u64_u32 and transcendentals are quarter rate).
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic VALU work per Heston FP32 path-step (lane-instructions of the
-# minimal code path, counted from the kernel's gfx950 ISA):
-#   2.06 u64 draws x 30 (PCG64DXSM: DXSM output 13 + 128x64-bit LCG step 17)
-#   2 normals x 10 (ziggurat fast path: layer/threshold test, u32->f32, scale, sign)
-#   1 stepper x 14 (FP32 full-truncation Euler in log space: 6 FMA, sqrt, max, ...)
-OPS_PER_STEP_HESTON32 = 2.06 * 30 + 2 * 10 + 14
+# Algorithmic VALU work per Heston FP32 path-step, in lane-cycles (quarter-
+# rate instructions count 4), for the minimal code path of the hot loop:
+#   per u64 draw  PCG64DXSM: 13 quarter-rate multiplies (DXSM output 6, the
+#                 128x64-bit LCG step 7) + 12 full-rate xor/shift/add-carry = 64
+#                 ziggurat fast path: index, 32-bit test, u32->f32, scale     =  5
+#   draws/step    2 normals x 1.0296 draws per normal (the ziggurat's draw rate)
+#   per step      full-truncation Euler in log space: 12 full-rate + sqrt (4) = 16
+DRAWS_PER_HESTON_STEP = 2 * 1.0296
+# HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+# scaled by the same counters on a 256 MiB copy); None until measured.
+TRAFFIC_BYTES = None
+OPS_PER_STEP_HESTON32 = DRAWS_PER_HESTON_STEP * (64 + 5) + 16
 
 
 def _sync():
@@ -84,6 +92,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip Merton and LSMC")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with one 256 MiB device-to-device copy (torch), the reference "
+                         "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build "
@@ -133,9 +144,9 @@ def main():
     }
     achieved = OPS_PER_STEP_HESTON32 * ps / (k32 * 1e-3) / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TOPS,
-                       "unit": "Tlane-op/s", "frac": achieved / PEAK_VALU_TOPS,
-                       "traffic": None,
-                       "kernel": "terminal_payoff_kernel<Heston32, CallPayoff>",
+                       "unit": "T lane-cycles/s", "frac": achieved / PEAK_VALU_TOPS,
+                       "traffic": TRAFFIC_BYTES,
+                       "kernel": "terminal_feed_kernel<HestonSrc<fp32>, Heston32, CallPayoff>",
                        "ops_per_path_step": OPS_PER_STEP_HESTON32,
                        "kernel_ms": k32}
 
@@ -178,6 +189,14 @@ def main():
             lsmc[str(n)] = {"fp32_ms": tl * 1e3, "fp32_price": price, "fp32_std_err": se,
                             "fp64_ms": tl64 * 1e3, "fp64_price": p64, "fp64_std_err": se64}
         out["lsmc_ms"] = lsmc
+
+    if args.calib_copy:
+        import torch
+        x = torch.empty(1 << 26, dtype=torch.float32, device="cuda")  # 256 MiB
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = 1 << 28
 
     print(json.dumps(out))
 

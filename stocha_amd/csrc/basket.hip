@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void basket_kernel(const BasketDev bd, cons
     for (uint64_t k = 0; k < total;) {
       const uint64_t u = g.next(inc);
       Real x;
-      if (zf.feed(u, g, inc, &zig, x)) {
+      if (zf.feed(u, &zig, x)) {
         W[a][threadIdx.x] += x;
         a = (a + 1 == d) ? 0 : a + 1;
         ++k;

@@ -54,30 +54,19 @@ __device__ __forceinline__ T bwave_sum(T v) {
   return v;
 }
 
-template <class Drv, class Const>
-__device__ __forceinline__ double batch_path(const Const& c, const StreamDesc& sd, uint64_t path,
-                                             uint32_t steps, const ZigLds* zig) {
-  const U128 inc = sd.inc;
-  Pcg g{stream_at(&sd, path)};
-  Drv d(c);
-  for (;;) {
-    const uint64_t u = g.next(inc);
-    if (d.feed(u, g, inc, zig, steps)) break;
-  }
-  return d.m.terminal();
-}
 
 template <int kBlock>
 __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchInstDev* __restrict__ inst,
                                                        const BatchTile* __restrict__ tiles,
                                                        const uint32_t n_tiles,
                                                        const StreamDesc* __restrict__ descs,
-                                                       const uint32_t paths_per_lane,
                                                        uint32_t* __restrict__ counter,
                                                        double* __restrict__ tile_sums) {
   __shared__ ZigLds zig;
   __shared__ uint32_t cur;
   __shared__ double red[kBlock / 64][2];
+  // step-input rings: 32 bytes per lane (8 GBM/Heston normals, 4 Merton entries)
+  __shared__ float2 qbuf[4][kBlock];
   zig_load(&zig);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (;;) {
@@ -90,16 +79,24 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchInstDev* __res
     const BatchInstDev& in = inst[tl.inst];
     const StreamDesc& sd = descs[in.desc];
     double s1 = 0.0, s2 = 0.0;
-    for (uint32_t j = 0; j < paths_per_lane; ++j) {
+    const U128 inc = sd.inc;
+    // every lane runs a path (lanes past the tile repeat its first path) so
+    // that the queue schedule sees whole wavefronts; only valid lanes count
+    for (uint32_t j = 0; j * kBlock < tl.count; ++j) {
       const uint32_t k = j * kBlock + threadIdx.x;
-      if (k < tl.count) {
-        double st;
-        if (in.model == kGbm)
-          st = batch_path<GbmDrv<Gbm32>>(in.g, sd, tl.path0 + k, in.steps, &zig);
-        else if (in.model == kHeston)
-          st = batch_path<HestonDrv<Heston32>>(in.h, sd, tl.path0 + k, in.steps, &zig);
-        else
-          st = batch_path<MertonDrv<Merton32>>(in.m, sd, tl.path0 + k, in.steps, &zig);
+      const bool valid = k < tl.count;
+      Pcg g{stream_at(&sd, tl.path0 + (valid ? k : 0))};
+      double st;
+      if (in.model == kGbm)
+        st = queue_path<GbmSrc<true>, Gbm32, 8, kBlock>(in.g, g, inc, in.steps, &zig,
+                                                        (GbmSrc<true>::Rec(*)[kBlock])qbuf);
+      else if (in.model == kHeston)
+        st = queue_path<HestonSrc<true>, Heston32, 8, kBlock>(
+            in.h, g, inc, in.steps, &zig, (HestonSrc<true>::Rec(*)[kBlock])qbuf);
+      else
+        st = queue_path<MertonSrc<true, Merton32>, Merton32, 4, kBlock>(
+            in.m, g, inc, in.steps, &zig, (MertonSrc<true, Merton32>::Rec(*)[kBlock])qbuf);
+      if (valid) {
         double pay = st - in.strike;
         pay = pay > 0.0 ? pay : 0.0;
         s1 += pay;
@@ -171,7 +168,7 @@ static void batch_enqueue(BatchPlan& P) {
   batch_reset_kernel<<<1, 1, 0, P.stream>>>(P.counter.as<uint32_t>());
   batch_kernel<kBlock><<<P.grid, kBlock, 0, P.stream>>>(
       P.inst.as<BatchInstDev>(), P.tiles.as<BatchTile>(), P.n_tiles, P.descs.as<StreamDesc>(),
-      P.paths_per_lane, P.counter.as<uint32_t>(), P.tile_sums.as<double>());
+      P.counter.as<uint32_t>(), P.tile_sums.as<double>());
   batch_reduce_kernel<<<(P.n_inst + 255) / 256, 256, 0, P.stream>>>(
       P.first.as<uint32_t>(), P.tile_ids.as<uint32_t>(), P.n_inst, P.tile_sums.as<double>(),
       P.out.as<double>());

@@ -23,6 +23,42 @@ Globals& globals() {
   return g;
 }
 
+int workspace(size_t dev_bytes, size_t host_bytes, Workspace** out, const Err& err) {
+  static std::vector<Workspace> per_dev;
+  int dev = 0;
+  STOCHA_HIP(hipGetDevice(&dev));
+  if ((size_t)dev >= per_dev.size()) per_dev.resize(dev + 1);
+  Workspace& w = per_dev[dev];
+  if (!w.counter) {
+    STOCHA_HIP(hipMalloc((void**)&w.counter, sizeof(unsigned) * kCounters));
+    STOCHA_HIP(hipMemset(w.counter, 0, sizeof(unsigned) * kCounters));
+  }
+  if (dev_bytes > w.dev_bytes) {
+    if (w.dev) STOCHA_HIP(hipFree(w.dev));
+    w.dev = nullptr;
+    w.dev_bytes = 0;
+    // round up to 1 MiB so small growth does not reallocate every call
+    const size_t want = (dev_bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+    hipError_t e = hipMalloc(&w.dev, want);
+    if (e != hipSuccess) {
+      w.dev = nullptr;
+      return err.set(STOCHA_ENOMEM, "device allocation of %zu bytes failed (%s)", want,
+                     hipGetErrorName(e));
+    }
+    w.dev_bytes = want;
+  }
+  if (host_bytes > w.host_bytes) {
+    if (w.host) STOCHA_HIP(hipHostFree(w.host));
+    w.host = nullptr;
+    w.host_bytes = 0;
+    const size_t want = host_bytes < 4096 ? 4096 : host_bytes;
+    STOCHA_HIP(hipHostMalloc((void**)&w.host, want, hipHostMallocDefault));
+    w.host_bytes = want;
+  }
+  *out = &w;
+  return STOCHA_OK;
+}
+
 GbmConst gbm_const(double s0, double mu, double sigma, double t, uint64_t steps, double q) {
   GbmConst c;
   const double dt = t / (double)steps;

@@ -1,98 +1,138 @@
-// drivers.hpp -- per-model draw-feed drivers: each consumes one u64 per
-// call and advances the path through the model's per-step grammar of draws
-// (GBM: one normal; Heston: two normals, heston.rs:60-61; Merton: a normal,
-// one uniform, and a jump normal when the uniform is below lambda*dt,
-// jump_diffusion.rs:75-85).  feed() returns true when the last step is done;
-// the model's terminal() then gives S_T.
+// drivers.hpp -- the queue schedule: per-model step-input sources fed one u64
+// per call, following each model's per-step grammar of draws (GBM: one
+// normal; Heston: two normals, heston.rs:60-61; Merton: a normal, one
+// uniform, and a jump normal when the uniform is below lambda*dt,
+// jump_diffusion.rs:75-85), and queue_path, which runs a whole wavefront of
+// paths through them.
 #pragma once
 #include "models.hpp"
 #include "zigsm.hpp"
 
 namespace stocha {
 
-template <class M>
-struct GbmDrv {
-  using Real = typename ZigFeed<M::kFp32>::Real;
-  M m;
-  ZigFeed<M::kFp32> zf;
-  uint32_t done = 0;
+// Step-input sources for queue_path.  A source turns the lane's draws, one
+// per call, into queue entries; Src::kPerStep entries make one step's input.
+// feed(u, z, rec, active) consumes u only when `active` (state unchanged
+// otherwise) and returns true when rec holds a new entry.
+//
+// GBM (gbm.rs:60-66) and Heston (heston.rs:92-107) take only normals, so the
+// entries are single normals: 1 per GBM step, 2 per Heston step (dw1 first,
+// then the independent dz).
+template <bool kFp32, uint32_t kPer>
+struct NormalSrc {
+  using Real = typename ZigFeed<kFp32>::Real;
+  using Rec = Real;
+  static constexpr uint32_t kPerStep = kPer;
+  static constexpr int kWaves = kFp32 ? 5 : 4;  // occupancy target of the fused kernel
+  ZigFeed<kFp32> zf;
   template <class C>
-  __device__ __forceinline__ explicit GbmDrv(const C& c) : m(c, false) {}
-  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
-                                       uint32_t steps) {
-    Real x;
-    if (zf.feed(u, g, inc, z, x)) {
-      m.apply(x);
-      return ++done == steps;
+  __device__ __forceinline__ explicit NormalSrc(const C&) {}
+  __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* z, Rec& r, bool active) {
+    return zf.feed(u, z, r, active);
+  }
+  template <uint32_t kSlots, int kCols, class M>
+  __device__ static __forceinline__ void apply(M& m, const Rec (*q)[kCols], uint32_t head) {
+    if constexpr (kPer == 1) {
+      m.apply(q[head][threadIdx.x]);
+    } else {
+      m.apply(q[head][threadIdx.x], q[(head + 1) & (kSlots - 1)][threadIdx.x]);
     }
-    return false;
   }
 };
+template <bool kFp32>
+using GbmSrc = NormalSrc<kFp32, 1>;
+template <bool kFp32>
+using HestonSrc = NormalSrc<kFp32, 2>;
 
-template <class M>
-struct HestonDrv {
-  using Real = typename ZigFeed<M::kFp32>::Real;
-  M m;
-  ZigFeed<M::kFp32> zf;
-  uint32_t done = 0;
-  bool have = false;
-  Real dw1 = 0;
-  template <class C>
-  __device__ __forceinline__ explicit HestonDrv(const C& c) : m(c, false) {}
-  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
-                                       uint32_t steps) {
-    Real x;
-    if (zf.feed(u, g, inc, z, x)) {
-      if (!have) {
-        dw1 = x;
-        have = true;
-        return false;
-      }
-      have = false;
-      m.apply(dw1, x);
-      return ++done == steps;
-    }
-    return false;
-  }
-};
-
-// Merton's per-step grammar: normal z, then one uniform (always), then a
-// jump normal when the uniform is below lambda*dt (jump_diffusion.rs:75-85).
-template <class M>
-struct MertonDrv {
-  using Real = typename ZigFeed<M::kFp32>::Real;
-  M m;
-  ZigFeed<M::kFp32> zf;
-  uint32_t done = 0;
-  int phase = 0;  // 0: drawing z, 1: next draw is the jump uniform, 2: drawing z_j
+// Merton: normal z, one uniform, a jump normal when the uniform is below
+// lambda*dt (jump_diffusion.rs:75-85); one entry per step carrying z and the
+// jump return (0 without a jump).
+template <bool kFp32, class M>
+struct MertonSrc {
+  using Real = typename ZigFeed<kFp32>::Real;
+  struct Rec {
+    Real z, jr;
+  };
+  static constexpr uint32_t kPerStep = 1;
+  static constexpr int kWaves = 4;
+  ZigFeed<kFp32> zf;
+  uint32_t phase = 0;  // 0: z next, 1: uniform next, 2: jump normal next
   Real zz = 0;
+  double p;  // lambda * dt
+  M jm;      // for jump_return()
   template <class C>
-  __device__ __forceinline__ explicit MertonDrv(const C& c) : m(c, false) {}
-  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc, const ZigLds* z,
-                                       uint32_t steps) {
-    if (phase == 1) {
-      const double uf = (double)(u >> 11) * 0x1p-53;
-      if (uf < m.c.jump_prob) {
-        phase = 2;
-        return false;
-      }
-      phase = 0;
-      m.apply(zz, false, (Real)0);
-      return ++done == steps;
-    }
+  __device__ __forceinline__ explicit MertonSrc(const C& c) : p(c.jump_prob), jm(c, false) {}
+  __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* z, Rec& r, bool active) {
     Real x;
-    if (zf.feed(u, g, inc, z, x)) {
+    const bool uni = phase == 1;
+    const bool got = zf.feed(u, z, x, active && !uni);
+    const bool jump = (double)(u >> 11) * 0x1p-53 < p;  // next_f64 (pcg64.rs:37-39)
+    bool out = false;
+    if (active && uni) {
+      phase = jump ? 2u : 0u;
+      r.z = zz;
+      r.jr = (Real)0;
+      out = !jump;
+    } else if (got) {
       if (phase == 0) {
         zz = x;
         phase = 1;
-        return false;
+      } else {
+        phase = 0;
+        r.z = zz;
+        r.jr = jm.jump_return(x);
+        out = true;
       }
-      phase = 0;
-      m.apply(zz, true, x);
-      return ++done == steps;
     }
-    return false;
+    return out;
+  }
+  template <uint32_t kSlots, int kCols, class MM>
+  __device__ static __forceinline__ void apply(MM& m, const Rec (*q)[kCols], uint32_t head) {
+    const Rec r = q[head][threadIdx.x];
+    m.apply_jr(r.z, r.jr);
   }
 };
+
+// One path simulated with the queue schedule.  Each lane feeds its draws into
+// src and appends the entries to its column of q, a ring of kSlots (a power of
+// two) holding at most kSlots - 1 entries; before each step the wave draws
+// until every lane holds a step's input (lanes that have one keep drawing
+// ahead while they have room), then applies the step on all lanes together.
+// The fill loop is branch-free apart from the ziggurat's slow block: every
+// lane computes the output and the next generator state, a lane with a full
+// ring keeps its old state, and the entry is stored unconditionally into the
+// first free slot (one slot is always free, so no live entry is overwritten).
+// Must be called by whole wavefronts (every lane, same steps).
+template <class Src, class Model, uint32_t kSlots, int kCols, class Const>
+__device__ __forceinline__ double queue_path(const Const& c, Pcg& g, const U128 inc,
+                                             uint32_t steps, const ZigLds* zig,
+                                             typename Src::Rec (*q)[kCols]) {
+  static_assert((kSlots & (kSlots - 1)) == 0 && kSlots > Src::kPerStep, "ring size");
+  constexpr uint32_t kCap = kSlots - 1;
+  constexpr uint32_t kNeed = Src::kPerStep;
+  using Rec = typename Src::Rec;
+  Src src(c);
+  Model m(c, false);
+  uint32_t head = 0, cnt = 0;
+  for (uint32_t step = 0; step < steps; ++step) {
+    // bottom-tested: one draw per step is always taken (by lanes with room),
+    // which keeps the loop-carried state in place across the test
+    do {
+      const bool active = cnt < kCap;
+      const uint64_t u = g.output();
+      const U128 sn = g.stepped(inc);
+      g.s.lo = active ? sn.lo : g.s.lo;
+      g.s.hi = active ? sn.hi : g.s.hi;
+      Rec r;
+      const bool got = src.feed(u, zig, r, active);
+      q[(head + cnt) & (kSlots - 1)][threadIdx.x] = r;
+      cnt += got ? 1u : 0u;
+    } while (__any(cnt < kNeed));
+    Src::template apply<kSlots, kCols>(m, q, head);
+    head = (head + kNeed) & (kSlots - 1);
+    cnt -= kNeed;
+  }
+  return m.terminal();
+}
 
 }  // namespace stocha

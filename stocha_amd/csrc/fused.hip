@@ -5,16 +5,18 @@
 // Each thread steps whole paths (grid-stride over path indices, every path on
 // its own block-split stream), evaluates the payoff on S_T and accumulates
 // FP64 sums.  Reduction: butterfly over the 64-lane wavefront, LDS across the
-// block's waves, one partial per block, then a single-block kernel sums the
-// partials in block order.  The grid size is a function of n_paths only, so
-// the summation order -- and the result -- is the same on every run.
+// block's waves, one partial per block; the last block to finish (a counter
+// in the workspace) sums the partials in a fixed order -- thread t over
+// blocks t, t+256, ..., then the same butterfly and wave order -- so the
+// result does not depend on which block finishes last.  The grid size is a
+// function of n_paths only, so the summation order -- and the result -- is
+// the same on every run.
 //
 // Two schedules draw the same numbers:
 //   kScheduleLoop  per-normal ziggurat loop (the reference's control flow)
-//   kScheduleFeed  flat per-lane state machine (zigsm.hpp): every iteration
-//                  draws one u64 on every lane; a lane that finishes a path
-//                  accumulates its payoff and starts its next path inside the
-//                  same loop, so lanes never wait on each other.
+//   kScheduleFeed  per-lane draw-feed state machine (zigsm.hpp) filling a
+//                  small LDS queue of step inputs; the wave applies each step
+//                  with all lanes together (terminal_feed_kernel).
 #include "host.hpp"
 #include "drivers.hpp"
 
@@ -79,10 +81,65 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Block sums of acc[0..na) into red, then red's wave order (thread i < na
+// gets the block total of slot i).
+__device__ __forceinline__ double block_total(const double* acc, int na,
+                                              double (*red)[kMaxAcc]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) {
+    if (i < na) {
+      const double w = wave_sum(acc[i]);
+      if (lane == 0) red[wave][i] = w;
+    }
+  }
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x < na)
+    for (int w = 0; w < kFusedBlock / 64; ++w) s += red[w][threadIdx.x];
+  return s;
+}
+
+// End of a fused kernel: store the block partial; the last block to arrive
+// reduces all partials in a fixed order into out[0..na) and re-zeroes the
+// counter (every block has incremented it by then).  The partials are
+// written and read with agent-scope relaxed atomics (sc1 loads/stores,
+// coherent across the XCDs' L2s) and ordered by s_waitcnt, so no block pays
+// for an L2 write-back / invalidate (__threadfence).
+__device__ __forceinline__ void finish_sums(const double* acc, int na, double (*red)[kMaxAcc],
+                                            double* partials, unsigned* __restrict__ counter,
+                                            double* __restrict__ out) {
+  __shared__ bool last;
+  const double s = block_total(acc, na, red);
+  if (threadIdx.x < na)
+    __hip_atomic_store(&partials[(uint64_t)blockIdx.x * na + threadIdx.x], s, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_s_waitcnt(0);  // the partial store has completed
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  double tot[kMaxAcc];
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) tot[i] = 0.0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kFusedBlock) {
+#pragma unroll
+    for (int i = 0; i < kMaxAcc; ++i)
+      if (i < na)
+        tot[i] += __hip_atomic_load(&partials[(uint64_t)b * na + i], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();  // red is reused
+  const double t = block_total(tot, na, red);
+  if (threadIdx.x < na) out[threadIdx.x] = t;
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
 template <class Model, class Const, bool kFp32, class Payoff>
 __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
     const Const mc, const StreamDesc sd, const uint64_t n_paths, const uint64_t steps,
-    const Payoff pay, double* __restrict__ partials) {
+    const Payoff pay, double* __restrict__ partials, unsigned* __restrict__ counter,
+    double* __restrict__ out) {
   __shared__ ZigLds zig;
   __shared__ double red[kFusedBlock / 64][kMaxAcc];
   zig_load(&zig);
@@ -97,30 +154,29 @@ __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
     const double st = terminal_value<Model, Const, kFp32>(mc, sd, p, steps, &zig);
     pay.add(st, acc);
   }
-
-  const int na = pay.n_acc();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < kMaxAcc; ++i) {
-    if (i < na) {
-      const double w = wave_sum(acc[i]);
-      if (lane == 0) red[wave][i] = w;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < na) {
-    double s = 0.0;
-    for (int w = 0; w < kFusedBlock / 64; ++w) s += red[w][threadIdx.x];
-    partials[(uint64_t)blockIdx.x * na + threadIdx.x] = s;
-  }
+  finish_sums(acc, pay.n_acc(), red, partials, counter, out);
 }
 
-// Flat state-machine kernel: one draw per lane per iteration.
-template <class Drv, class Const, class Payoff>
-__global__ __launch_bounds__(kFusedBlock) void terminal_feed_kernel(
+// Queue kernel (the FEED schedule).  Each lane feeds its own draws into its
+// model's source and appends the step inputs to a small ring in LDS (kQueue
+// slots per lane, queue_path).  Before every step the wave draws until each
+// lane holds one step's input -- lanes that already do keep drawing ahead
+// while their ring has room -- and then all lanes apply the step together,
+// so the step arithmetic never runs with half the wave masked.  Simulated on
+// the draw-count distribution of the ziggurat, a ring of 8 Heston normals
+// costs ~2.14 draws per step against the 2.06 minimum.
+// Occupancy target per source (Src::kWaves): the FP32 GBM/Heston kernels fit
+// 5 waves per SIMD (<= 96 VGPRs, 29 KB of LDS per block) without spilling;
+// the others need more registers and run 4.
+template <class Src, class Model, class Const, class Payoff, int kQueue>
+__global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
+    Src::kWaves))) void terminal_feed_kernel(
     const Const mc, const StreamDesc sd, const uint64_t n_paths, const uint32_t steps,
-    const Payoff pay, double* __restrict__ partials) {
+    const Payoff pay, double* __restrict__ partials, unsigned* __restrict__ counter,
+    double* __restrict__ out) {
+  using Rec = typename Src::Rec;
   __shared__ ZigLds zig;
+  __shared__ Rec q[kQueue][kFusedBlock];
   __shared__ double red[kFusedBlock / 64][kMaxAcc];
   zig_load(&zig);
   __syncthreads();
@@ -129,39 +185,16 @@ __global__ __launch_bounds__(kFusedBlock) void terminal_feed_kernel(
 #pragma unroll
   for (int i = 0; i < kMaxAcc; ++i) acc[i] = 0.0;
 
+  const U128 inc = sd.inc;
   const uint64_t stride = (uint64_t)gridDim.x * kFusedBlock;
-  uint64_t p = (uint64_t)blockIdx.x * kFusedBlock + threadIdx.x;
-  if (p < n_paths) {
-    const U128 inc = sd.inc;
-    Pcg g{stream_at(&sd, p)};
-    Drv d(mc);
-    for (;;) {
-      const uint64_t u = g.next(inc);
-      if (d.feed(u, g, inc, &zig, steps)) {
-        pay.add(d.m.terminal(), acc);
-        p += stride;
-        if (p >= n_paths) break;
-        g.s = stream_at(&sd, p);
-        d = Drv(mc);
-      }
-    }
+  for (uint64_t base = (uint64_t)blockIdx.x * kFusedBlock; base < n_paths; base += stride) {
+    const uint64_t p = base + threadIdx.x;
+    const bool valid = p < n_paths;
+    Pcg g{stream_at(&sd, valid ? p : 0)};
+    const double st = queue_path<Src, Model, kQueue, kFusedBlock>(mc, g, inc, steps, &zig, q);
+    if (valid) pay.add(st, acc);
   }
-
-  const int na = pay.n_acc();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < kMaxAcc; ++i) {
-    if (i < na) {
-      const double w = wave_sum(acc[i]);
-      if (lane == 0) red[wave][i] = w;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < na) {
-    double s = 0.0;
-    for (int w = 0; w < kFusedBlock / 64; ++w) s += red[w][threadIdx.x];
-    partials[(uint64_t)blockIdx.x * na + threadIdx.x] = s;
-  }
+  finish_sums(acc, pay.n_acc(), red, partials, counter, out);
 }
 
 // Sums partials[b][i] over b in block order; one thread per accumulator
@@ -200,29 +233,32 @@ static unsigned fused_grid(uint64_t n_paths) {
   return (unsigned)b;
 }
 
-template <class Model, class Drv, class Const, bool kFp32, class Payoff>
+template <class Model, class Src, int kQueue, class Const, bool kFp32, class Payoff>
 static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
                         const Payoff& pay, int na, double* h_out, int schedule, hipStream_t st,
                         const Err& err) {
   if (steps > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps must be below 2^32");
   const unsigned grid = fused_grid(n_paths);
-  DevBuf part, out;
-  if (int rc = part.alloc(sizeof(double) * grid * na, err)) return rc;
-  if (int rc = out.alloc(sizeof(double) * na, err)) return rc;
+  Workspace* ws = nullptr;
+  const size_t part_bytes = sizeof(double) * grid * na;
+  if (int rc = workspace(part_bytes + sizeof(double) * kMaxAcc, sizeof(double) * kMaxAcc, &ws,
+                         err))
+    return rc;
+  double* part = static_cast<double*>(ws->dev);
+  double* out = part + (size_t)grid * na;
   KernelTimer timer(st);
   timer.start();
   if (schedule == kScheduleLoop)
-    terminal_payoff_kernel<Model, Const, kFp32, Payoff>
-        <<<grid, kFusedBlock, 0, st>>>(mc, sd, n_paths, steps, pay, part.as<double>());
+    terminal_payoff_kernel<Model, Const, kFp32, Payoff><<<grid, kFusedBlock, 0, st>>>(
+        mc, sd, n_paths, steps, pay, part, ws->counter, out);
   else
-    terminal_feed_kernel<Drv, Const, Payoff>
-        <<<grid, kFusedBlock, 0, st>>>(mc, sd, n_paths, (uint32_t)steps, pay, part.as<double>());
+    terminal_feed_kernel<Src, Model, Const, Payoff, kQueue><<<grid, kFusedBlock, 0, st>>>(
+        mc, sd, n_paths, (uint32_t)steps, pay, part, ws->counter, out);
   STOCHA_HIP(hipGetLastError());
   timer.stop(1);
-  reduce_partials_kernel<<<1, 256, 0, st>>>(part.as<double>(), (int)grid, na, out.as<double>());
-  STOCHA_HIP(hipGetLastError());
-  STOCHA_HIP(hipMemcpyAsync(h_out, out.p, sizeof(double) * na, hipMemcpyDeviceToHost, st));
+  STOCHA_HIP(hipMemcpyAsync(ws->host, out, sizeof(double) * na, hipMemcpyDeviceToHost, st));
   STOCHA_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < na; ++i) h_out[i] = ws->host[i];
   return STOCHA_OK;
 }
 
@@ -237,18 +273,18 @@ int launch_fused(int model, int precision, int schedule, const void* mconst,
   switch (model) {
     case kGbm: {
       const GbmConst& c = *(const GbmConst*)mconst;
-      return f32 ? run_terminal<Gbm32, GbmDrv<Gbm32>, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
-                 : run_terminal<Gbm64, GbmDrv<Gbm64>, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
+      return f32 ? run_terminal<Gbm32, GbmSrc<true>, 8, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+                 : run_terminal<Gbm64, GbmSrc<false>, 8, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     case kHeston: {
       const HestonConst& c = *(const HestonConst*)mconst;
-      return f32 ? run_terminal<Heston32, HestonDrv<Heston32>, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
-                 : run_terminal<Heston64, HestonDrv<Heston64>, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
+      return f32 ? run_terminal<Heston32, HestonSrc<true>, 8, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+                 : run_terminal<Heston64, HestonSrc<false>, 8, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     case kMerton: {
       const MertonConst& c = *(const MertonConst*)mconst;
-      return f32 ? run_terminal<Merton32, MertonDrv<Merton32>, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
-                 : run_terminal<Merton64, MertonDrv<Merton64>, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
+      return f32 ? run_terminal<Merton32, MertonSrc<true, Merton32>, 4, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+                 : run_terminal<Merton64, MertonSrc<false, Merton64>, 4, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     default:
       return err.set(STOCHA_EINVAL, "unknown model id %d", model);
@@ -266,7 +302,7 @@ int run_greeks_pathwise(const GbmConst& gc, const StreamDesc& sd, uint64_t n_pat
   pay.sigma = sigma;
   pay.is_call = is_call;
   double h[2];
-  int rc = run_terminal<Gbm64, GbmDrv<Gbm64>, GbmConst, false>(gc, sd, n_paths, steps, pay, 2, h,
+  int rc = run_terminal<Gbm64, GbmSrc<false>, 8, GbmConst, false>(gc, sd, n_paths, steps, pay, 2, h,
                                                                kScheduleFeed, st, err);
   *delta_sum = h[0];
   *vega_sum = h[1];

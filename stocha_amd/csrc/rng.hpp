@@ -64,20 +64,45 @@ struct StreamDesc {
 struct Pcg {
   U128 s;
 
-  __device__ __forceinline__ uint64_t next(const U128 inc) {
-    // DXSM output of the pre-step state (rand_pcg output_dxsm)
+  // DXSM output of the current (pre-step) state (rand_pcg output_dxsm)
+  __device__ __forceinline__ uint64_t output() const {
     uint64_t hi = s.hi;
-    uint64_t lo = s.lo | 1ull;
+    const uint64_t lo = s.lo | 1ull;
     hi ^= hi >> 32;
     hi *= kPcgMul;
     hi ^= hi >> 48;
     hi *= lo;
-    // LCG step: s = s*M + inc (mod 2^128), M < 2^64
-    U128 n;
-    n.lo = s.lo * kPcgMul;
-    n.hi = __umul64hi(s.lo, kPcgMul) + s.hi * kPcgMul;
-    s = add128(n, inc);
     return hi;
+  }
+  // LCG step: s*M + inc (mod 2^128), M < 2^64, on 32-bit limbs so that the
+  // 32x32+64 multiply-adds (v_mad_u64_u32) and the carry chain map directly:
+  // lo*M = t + (u << 32) + ... needs 4 multiply-adds, hi*M mod 2^64 needs
+  // one more and two 32-bit products.
+  __device__ __forceinline__ U128 stepped(const U128 inc) const {
+    constexpr uint32_t m0 = (uint32_t)kPcgMul, m1 = (uint32_t)(kPcgMul >> 32);
+    const uint32_t l0 = (uint32_t)s.lo, l1 = (uint32_t)(s.lo >> 32);
+    const uint32_t h0 = (uint32_t)s.hi, h1 = (uint32_t)(s.hi >> 32);
+    const uint64_t t = (uint64_t)l0 * m0;
+    const uint64_t u = (uint64_t)l0 * m1 + (t >> 32);
+    const uint64_t w = (uint64_t)l1 * m0 + (uint32_t)u;
+    // bits 64..127 of lo*M (no overflow: (2^32-1)^2 + 2 (2^32-1) < 2^64)
+    const uint64_t x = (uint64_t)l1 * m1 + ((u >> 32) + (w >> 32));
+    const uint64_t a = (uint64_t)h0 * m0 + x;  // wraps mod 2^64, as wanted
+    const uint32_t cross = h0 * m1 + h1 * m0;
+    uint32_t c0, c1, c2;
+    const uint32_t n0 = __builtin_addc((uint32_t)t, (uint32_t)inc.lo, 0u, &c0);
+    const uint32_t n1 = __builtin_addc((uint32_t)w, (uint32_t)(inc.lo >> 32), c0, &c1);
+    const uint32_t n2 = __builtin_addc((uint32_t)a, (uint32_t)inc.hi, c1, &c2);
+    const uint32_t n3 = (uint32_t)(a >> 32) + cross + (uint32_t)(inc.hi >> 32) + c2;
+    U128 n;
+    n.lo = ((uint64_t)n1 << 32) | n0;
+    n.hi = ((uint64_t)n3 << 32) | n2;
+    return n;
+  }
+  __device__ __forceinline__ uint64_t next(const U128 inc) {
+    const uint64_t o = output();
+    s = stepped(inc);
+    return o;
   }
 
   // pcg64.rs:37-39: (u >> 11) * 2^-53, exact
@@ -102,17 +127,50 @@ __device__ __forceinline__ U128 stream_at(const StreamDesc* __restrict__ d, uint
   return s;
 }
 
+// u32 -> f32 of the high word of a draw.  LLVM folds uitofp(trunc(x >> 32))
+// back into a 64-bit conversion (six instructions on gfx950); the empty asm
+// makes the 32-bit operand opaque so a single v_cvt_f32_u32 is emitted.  (No
+// memory access is involved.)
+__device__ __forceinline__ float hi32_to_f32(uint64_t u) {
+  uint32_t h = (uint32_t)(u >> 32);
+  asm("" : "+v"(h));
+  return (float)h;
+}
+__device__ __forceinline__ float top24_to_f32(uint64_t u) {  // (u >> 40) as float, exact
+  uint32_t h = (uint32_t)(u >> 40);
+  asm("" : "+v"(h));
+  return (float)h;
+}
+
 // ---------------------------------------------------------------------------
-// Ziggurat tables staged in LDS.  Fast-path entry packs the integer
-// threshold and X[l] so one 16-byte LDS read serves the common case.
+// Ziggurat tables staged in LDS, indexed by the low 9 bits of a draw
+// (layer = u & 255, sign = bit 8, normal.rs:20-24): entry i holds layer
+// i & 255 with the sign of i folded into X, so the value needs no separate
+// negation ((-a) * b == -(a * b) exactly in IEEE arithmetic).
+//
+// Fast-path test (normal.rs:26-29): (u >> 11) < T[l]  <=>  u < T[l] << 11
+// (T < 2^53, no overflow).  With th = T >> 21, the high word of u decides it
+// unless it equals th: hi < th accepts, hi > th rejects, and the tie (one
+// draw in 2^32) is settled from the full T in the slow block.
 // ---------------------------------------------------------------------------
 struct ZigFast {
-  uint64_t t;  // accept iff (u >> 11) < t
-  double x;    // X[l]
+  uint32_t th;  // T[l] >> 21
+  uint32_t pad;
+  double x;  // +-X[l]
+};
+
+// FP32 form: x = hi32(u) * (X[l] * 2^-32) in FP32 (the top 32 of the 53 bits
+// of m = u >> 11), one 8-byte entry.
+struct ZigFast32 {
+  uint32_t th;  // T[l] >> 21
+  float xs;     // +-X[l] * 2^-32, rounded to FP32
 };
 
 struct ZigLds {
-  ZigFast fast[256];
+  ZigFast fast[512];
+  ZigFast32 fast32[512];
+  uint64_t t[256];    // T[l], exact fast-path threshold on m = u >> 11
+  float2 ydy32[256];  // FP32 wedge: (Y[l], Y[l+1] - Y[l])
   double x[257];
   double y[257];
 };
@@ -124,35 +182,91 @@ constexpr double kZigR = ZIG_R_VALUE;
 
 // Cooperative fill; caller must __syncthreads() afterwards.
 __device__ __forceinline__ void zig_load(ZigLds* z) {
-  for (int i = threadIdx.x; i < 257; i += blockDim.x) {
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    const int l = i & 255;
+    const double xv = (i & 256) ? -kZigX[l] : kZigX[l];
+    const uint32_t th = (uint32_t)(kZigT[l] >> 21);
+    z->fast[i].th = th;
+    z->fast[i].pad = 0u;
+    z->fast[i].x = xv;
+    z->fast32[i].th = th;
+    z->fast32[i].xs = (float)(xv * 0x1p-32);
     if (i < 256) {
-      z->fast[i].t = kZigT[i];
-      z->fast[i].x = kZigX[i];
+      z->t[i] = kZigT[i];
+      z->ydy32[i] = make_float2((float)kZigY[i], (float)(kZigY[i + 1] - kZigY[i]));
     }
-    z->x[i] = kZigX[i];
-    z->y[i] = kZigY[i];
+    if (i < 257) {
+      z->x[i] = kZigX[i];
+      z->y[i] = kZigY[i];
+    }
   }
 }
 
-// normal.rs:54-67 (always FP64: both forms consume the same draws).
-// Kept out of line (it runs for ~0.02% of attempts); the generator state goes
-// in and comes back by value so the caller's state stays in registers.
+// normal.rs:54-67, one (u1, u2) pair from draws a, b: returns whether the pair
+// is accepted and then the tail value x + R (x = -ln(u1) / R) in *t.
+// The accept test -2 ln(u2) >= x^2 is evaluated with the hardware log2 in
+// FP32 and decided there unless |d| = |-2 ln u2 - x^2| is inside a guard band
+// of 1e-5 (1 + |terms|); there the FP64 test is recomputed (the FP32 error is
+// below 2e-6 (1 + |terms|): FP32 rounding of u, log2 within a few ulp), so
+// the decision -- and the draws consumed -- are always the FP64 ones.
+__device__ __forceinline__ bool zig_tail_test(double u1, double u2) {
+  const float l1 = __builtin_amdgcn_logf((float)u1);  // log2, u >= 2^-53 is a normal float
+  const float l2 = __builtin_amdgcn_logf((float)u2);
+  const float x = l1 * (float)(-0.69314718055994531 / kZigR);
+  const float a = l2 * -1.3862943611198906f;
+  const float x2 = x * x;
+  const float d = a - x2;
+  if (fabsf(d) > 1e-5f * (1.f + a + x2)) return d >= 0.f;
+  const double xd = -log(u1) / kZigR;
+  return -2.0 * log(u2) >= xd * xd;
+}
+__device__ __forceinline__ bool zig_tail_pair64(uint64_t a, uint64_t b, double* t) {
+  const double u1 = (double)(a >> 11) * 0x1p-53;  // next_f64 (pcg64.rs:37-39)
+  const double u2 = (double)(b >> 11) * 0x1p-53;
+  if (u1 <= 0.0 || u2 <= 0.0) return false;
+  if (!zig_tail_test(u1, u2)) return false;
+  *t = -log(u1) / kZigR + kZigR;
+  return true;
+}
+// FP32 form: same decision; the value from the FP32 log (within ~1 ulp of
+// the FP64 value rounded to FP32).
+__device__ __forceinline__ bool zig_tail_pair32(uint64_t a, uint64_t b, float* t) {
+  const double u1 = (double)(a >> 11) * 0x1p-53;
+  const double u2 = (double)(b >> 11) * 0x1p-53;
+  if (u1 <= 0.0 || u2 <= 0.0) return false;
+  if (!zig_tail_test(u1, u2)) return false;
+  const float l1 = __builtin_amdgcn_logf((float)u1);
+  *t = fmaf(l1, (float)(-0.69314718055994531 / kZigR), (float)kZigR);
+  return true;
+}
+
+// The tail loop for the loop-form samplers.  Kept out of line (it runs for
+// 1/256 of attempts); the generator state goes in and comes back by value so
+// the caller's state stays in registers.
+template <class Real>
 struct TailOut {
-  double x;
+  Real x;
   U128 s;
 };
-__device__ __noinline__ TailOut zig_tail_impl(U128 s, const U128 inc) {
+template <class Real>
+__device__ __noinline__ TailOut<Real> zig_tail_impl(U128 s, const U128 inc) {
   Pcg g{s};
   for (;;) {
-    double u1 = g.next_f64(inc);
-    double u2 = g.next_f64(inc);
-    if (u1 <= 0.0 || u2 <= 0.0) continue;
-    double x = -log(u1) / kZigR;
-    if (-2.0 * log(u2) >= x * x) return TailOut{x + kZigR, g.s};
+    const uint64_t a = g.next(inc);
+    const uint64_t b = g.next(inc);
+    Real t;
+    bool ok;
+    if constexpr (sizeof(Real) == 4) {
+      ok = zig_tail_pair32(a, b, &t);
+    } else {
+      ok = zig_tail_pair64(a, b, &t);
+    }
+    if (ok) return TailOut<Real>{t, g.s};
   }
 }
-__device__ __forceinline__ double zig_tail(Pcg& g, const U128 inc) {
-  TailOut t = zig_tail_impl(g.s, inc);
+template <class Real>
+__device__ __forceinline__ Real zig_tail(Pcg& g, const U128 inc) {
+  const TailOut<Real> t = zig_tail_impl<Real>(g.s, inc);
   g.s = t.s;
   return t.x;
 }
@@ -162,23 +276,22 @@ __device__ __forceinline__ double zig_tail(Pcg& g, const U128 inc) {
 __device__ __forceinline__ double zig_normal64(Pcg& g, const U128 inc, const ZigLds* __restrict__ z) {
   for (;;) {
     const uint64_t u = g.next(inc);
-    const uint32_t layer = (uint32_t)u & 0xffu;
+    const uint32_t idx = (uint32_t)u & 0x1ffu, layer = idx & 0xffu;
     const uint64_t m = u >> 11;
-    const ZigFast f = z->fast[layer];
-    const double x = ((double)m * 0x1p-53) * f.x;
-    const bool neg = (u >> 8) & 1ull;
-    if (m < f.t) return neg ? -x : x;
+    const double x = ((double)m * 0x1p-53) * z->fast[idx].x;  // signed
+    if (m < z->t[layer]) return x;
     if (layer == 0) {
-      double tl = zig_tail(g, inc);
-      return neg ? -tl : tl;
+      const double tl = zig_tail<double>(g, inc);
+      return (idx & 256u) ? -tl : tl;
     }
     const double y = z->y[layer] + g.next_f64(inc) * (z->y[layer + 1] - z->y[layer]);
-    if (y < exp(-0.5 * x * x)) return neg ? -x : x;
+    if (y < exp(-0.5 * x * x)) return x;
   }
 }
 
-// FP32 form.  Same draws as zig_normal64; the returned value is the FP32
-// rounding of the top 32 bits of m times X[l] (relative error < 2^-23).
+// FP32 form.  Same draws as zig_normal64; the fast-path value is the top 32
+// bits of m times X[l] in FP32 (absolute error <= 2^-32 * X[l] < 1e-9 from
+// the dropped mantissa bits, plus FP32 rounding).
 // The wedge decision uses FP64 y and x (cheap: full-rate FP64 mul/add) and a
 // hardware exp2 in FP32; when |y - e| is within 1e-5*e of the boundary it
 // recomputes exp in FP64, so the accept/reject decision equals the FP64
@@ -186,20 +299,15 @@ __device__ __forceinline__ double zig_normal64(Pcg& g, const U128 inc, const Zig
 __device__ __forceinline__ float zig_normal32(Pcg& g, const U128 inc, const ZigLds* __restrict__ z) {
   for (;;) {
     const uint64_t u = g.next(inc);
-    const uint32_t layer = (uint32_t)u & 0xffu;
+    const uint32_t idx = (uint32_t)u & 0x1ffu, layer = idx & 0xffu;
     const uint64_t m = u >> 11;
-    const ZigFast f = z->fast[layer];
-    const bool neg = (u >> 8) & 1ull;
-    if (m < f.t) {
-      // m = u >> 11 has 53 bits; its top 32 are u >> 32.
-      float xf = (float)(uint32_t)(u >> 32) * ((float)f.x * 0x1p-32f);
-      return neg ? -xf : xf;
-    }
+    const float xf = hi32_to_f32(u) * z->fast32[idx].xs;  // signed
+    if (m < z->t[layer]) return xf;
     if (layer == 0) {
-      double tl = zig_tail(g, inc);
-      return neg ? -(float)tl : (float)tl;
+      const float tl = zig_tail<float>(g, inc);
+      return (idx & 256u) ? -tl : tl;
     }
-    const double x = ((double)m * 0x1p-53) * f.x;
+    const double x = ((double)m * 0x1p-53) * z->x[layer];
     const double y = z->y[layer] + g.next_f64(inc) * (z->y[layer + 1] - z->y[layer]);
     const float arg = (float)(-0.5 * x * x);
     const float e = __builtin_amdgcn_exp2f(arg * 1.44269504088896341f);
@@ -210,7 +318,7 @@ __device__ __forceinline__ float zig_normal32(Pcg& g, const U128 inc, const ZigL
     } else {
       accept = y < exp(-0.5 * x * x);
     }
-    if (accept) return neg ? -(float)x : (float)x;
+    if (accept) return xf;  // same FP32 value as the fast path and the feed schedule
   }
 }
 

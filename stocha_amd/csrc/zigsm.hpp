@@ -5,17 +5,26 @@
 // (zig_normal64/32), a lane that leaves the fast path (wedge: 1.4% of
 // attempts, layer 255: 0.4%) makes the whole wave execute the extra wedge draw
 // and the retry; with 64 lanes that happens on ~69% of normals, so the wave
-// pays ~2 PCG draws per normal.  Here every loop iteration draws exactly one
-// u64 on every active lane, and each lane decides what that draw is from its
-// own state:
-//   attempt  -> fast accept (normal done), tail (rare, resolved in place), or
-//               wedge pending (x kept)
-//   wedge    -> the draw is the wedge uniform: accept (normal done) or back to
-//               attempt
+// pays ~2 PCG draws per normal.  Here every call consumes exactly the one u64
+// it is given, and each lane decides what that draw is from its own state:
+//   0 attempt  -> fast accept (normal done), wedge pending, or tail pending
+//   1 wedge    -> the draw is the wedge uniform: accept, or back to attempt
+//   2, 3 tail  -> the draws are the tail sampler's (u1, u2) pair
 // A lane's draws are consumed in exactly the order of normal.rs:18-67, so
-// values are unchanged; only the wave's schedule differs.  The steppers layer
-// their own per-step grammar on top (Merton's jump uniform is a draw too).
+// values are unchanged; only the wave's schedule differs.  Since a call never
+// draws by itself, the caller's generator step is unconditional and can
+// overlap the LDS table reads.
+//
+// FP32 form: the fast-path test is the integer test (identical to FP64); the
+// wedge test is evaluated in FP32 and decided there unless |y - e| is within
+// a guard band of 4e-6, where the FP64 test of the reference is recomputed
+// from the stored draws (the FP32 evaluation errs by < 2e-6 absolute: y from
+// 24 bits of the uniform and FP32 table values, e = exp2(-x^2/2 log2 e) with
+// x from the top 32 bits of m), so every accept/reject decision is the FP64
+// one and both forms consume the same draws.
 #pragma once
+#include <type_traits>
+
 #include "rng.hpp"
 
 namespace stocha {
@@ -23,61 +32,86 @@ namespace stocha {
 template <bool kFp32>
 struct ZigFeed {
   using Real = typename std::conditional<kFp32, float, double>::type;
-  bool wedge = false;  // next draw is the wedge uniform of a pending candidate
-  bool negw = false;
-  uint32_t layer = 0;
-  double xw = 0.0;  // FP64 candidate (the wedge decision needs it exactly)
+  uint32_t state = 0;
+  uint32_t idx = 0;  // table index (layer | sign << 8) of a pending wedge / tail
+  uint64_t ua = 0;   // wedge: the attempt draw (x); tail: the pair's first draw
 
-  // Feed draw u (already taken from g).  Returns true when a normal is
-  // complete and stores it in out.  g is only used by the tail sampler,
-  // which draws its own pairs.
-  __device__ __forceinline__ bool feed(uint64_t u, Pcg& g, const U128 inc,
-                                       const ZigLds* __restrict__ z, Real& out) {
-    if (!wedge) {
-      const uint32_t l = (uint32_t)u & 0xffu;
-      const uint64_t m = u >> 11;
-      const ZigFast f = z->fast[l];
-      const bool neg = (u >> 8) & 1ull;
-      if (m < f.t) {
-        Real x;
-        if constexpr (kFp32)
-          x = (float)(uint32_t)(u >> 32) * ((float)f.x * 0x1p-32f);
-        else
-          x = ((double)m * 0x1p-53) * f.x;
-        out = neg ? -x : x;
-        return true;
-      }
-      if (l == 0) {
-        const double tl = zig_tail(g, inc);
-        out = neg ? -(Real)tl : (Real)tl;
-        return true;
-      }
-      wedge = true;
-      negw = neg;
-      layer = l;
-      xw = ((double)m * 0x1p-53) * f.x;
+  // Consumes draw u when `active` (state unchanged otherwise).  The common
+  // case is straight-line on every lane: one table read, a 32-bit test, the
+  // value; lanes not in the attempt state, or whose attempt is not a clear
+  // fast accept, take the one divergent block (feed_slow).  `out` is written
+  // on every call and holds a normal only when the call returns true.
+  __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* __restrict__ z, Real& out,
+                                       bool active = true) {
+    const uint32_t i = (uint32_t)u & 0x1ffu;
+    const uint32_t hi = (uint32_t)(u >> 32);
+    bool fast;
+    if constexpr (kFp32) {
+      const ZigFast32 f = z->fast32[i];
+      fast = hi < f.th;
+      out = hi32_to_f32(u) * f.xs;
+    } else {
+      const ZigFast f = z->fast[i];
+      fast = hi < f.th;
+      out = ((double)(u >> 11) * 0x1p-53) * f.x;
+    }
+    const bool take = active && state == 0 && fast;
+    if (active && !take) return feed_slow(u, z, out);
+    return take;
+  }
+
+  __device__ __forceinline__ bool feed_slow(uint64_t u, const ZigLds* __restrict__ z, Real& out) {
+    if (state == 0) {  // attempt that missed the 32-bit test
+      const uint32_t i = (uint32_t)u & 0x1ffu, l = i & 0xffu;
+      if ((u >> 11) < z->t[l]) return true;  // high-word tie, fast accept after all; out holds x
+      ua = u;
+      idx = i;
+      state = l == 0 ? 2u : 1u;
       return false;
     }
-    wedge = false;
-    const double y0 = z->y[layer], y1 = z->y[layer + 1];
-    const double y = y0 + ((double)(u >> 11) * 0x1p-53) * (y1 - y0);
-    bool accept;
+    if (state == 1) {  // wedge (normal.rs:36-42): u is the uniform
+      state = 0;
+      const uint32_t l = idx & 0xffu;
+      if constexpr (kFp32) {
+        const float2 yd = z->ydy32[l];
+        const float y = fmaf(top24_to_f32(u) * 0x1p-24f, yd.y, yd.x);
+        const float x = hi32_to_f32(ua) * z->fast32[idx].xs;
+        const float e = __builtin_amdgcn_exp2f(-0.72134752044448170f * x * x);
+        const float diff = y - e;
+        bool accept;
+        if (fabsf(diff) > 4e-6f) {
+          accept = diff < 0.f;
+        } else {
+          const double x64 = ((double)(ua >> 11) * 0x1p-53) * z->x[l];
+          const double y64 = z->y[l] + ((double)(u >> 11) * 0x1p-53) * (z->y[l + 1] - z->y[l]);
+          accept = y64 < exp(-0.5 * x64 * x64);
+        }
+        out = x;
+        return accept;
+      } else {
+        const double x = ((double)(ua >> 11) * 0x1p-53) * z->fast[idx].x;
+        const double y = z->y[l] + ((double)(u >> 11) * 0x1p-53) * (z->y[l + 1] - z->y[l]);
+        out = x;
+        return y < exp(-0.5 * x * x);
+      }
+    }
+    if (state == 2) {  // tail (normal.rs:54-67): the pair's first draw
+      ua = u;
+      state = 3;
+      return false;
+    }
+    state = 2;  // the pair's second draw
+    Real t;
+    bool ok;
     if constexpr (kFp32) {
-      // hardware exp2 in FP32; exact FP64 exp only near the boundary
-      const float e = __builtin_amdgcn_exp2f((float)(-0.5 * xw * xw) * 1.44269504088896341f);
-      const double diff = y - (double)e;
-      if (fabs(diff) > 1e-5 * (double)e)
-        accept = diff < 0.0;
-      else
-        accept = y < exp(-0.5 * xw * xw);
+      ok = zig_tail_pair32(ua, u, &t);
     } else {
-      accept = y < exp(-0.5 * xw * xw);
+      ok = zig_tail_pair64(ua, u, &t);
     }
-    if (accept) {
-      out = negw ? -(Real)xw : (Real)xw;
-      return true;
-    }
-    return false;
+    if (!ok) return false;
+    state = 0;
+    out = (idx & 256u) ? -t : t;
+    return true;
   }
 };
 
