@@ -38,25 +38,28 @@ T = 1.0
 SEED = 42
 
 # MI355X peaks: HBM 8 TB/s; VALU issue 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
-# = 39.3 T lane-cycles/s (FP32 and FP64 FMA are full rate on CDNA4; 32-bit
-# integer multiplies, v_mad_u64_u32 and transcendentals are quarter rate).
+# = 39.3 T lane-instructions/s (one wave64 VALU instruction per 4 clocks per
+# SIMD).  Measured on this kernel: kernel time = SQ_INSTS_VALU x 4 clocks per
+# SIMD, i.e. the 32-bit integer multiplies of PCG issue at the full rate too.
 PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic VALU work per Heston FP32 path-step, in lane-cycles (quarter-
-# rate instructions count 4), for the minimal code path of the hot loop:
-#   per u64 draw  PCG64DXSM: 13 quarter-rate multiplies (DXSM output 6, the
-#                 128x64-bit LCG step 7) + 12 full-rate xor/shift/add-carry = 64
-#                 ziggurat fast path: index, 32-bit test, u32->f32, scale     =  5
-#   draws/step    2 normals x 1.0296 draws per normal (the ziggurat's draw rate)
-#   per step      full-truncation Euler in log space: 12 full-rate + sqrt (4) = 16
+# Algorithmic VALU work per Heston FP32 path-step, in lane-instructions of the
+# minimal gfx950 code for each primitive:
+#   per u64 draw  PCG64DXSM DXSM output 11 (6 multiplies) + 128x64-bit LCG
+#                 step 14 (7 multiply-adds, 7 add-with-carry)              = 25
+#                 ziggurat fast path: index, 32-bit test, u32->f32, scale    =  5
+#   draws/step    2 normals x 1.0296 draws per normal (the ziggurat's rate)
+#   per step      full-truncation Euler in log space (6 FMA, max, sqrt, ...) = 14
 DRAWS_PER_HESTON_STEP = 2 * 1.0296
-# HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-# scaled by the same counters on a 256 MiB copy); None until measured.
-TRAFFIC_BYTES = None
-OPS_PER_STEP_HESTON32 = DRAWS_PER_HESTON_STEP * (64 + 5) + 16
-
-
+# HBM bytes per launch of that kernel: rocprofv3 FETCH_SIZE and WRITE_SIZE
+# (KiB per dispatch) scaled by the same counters on a 256 MiB device copy
+# (FETCH_SIZE reads 131092 KiB for 262144 KiB moved -> x1.9997; WRITE_SIZE
+# reads 262144 -> x1.0000): 221.25 KiB x 1.9997 + 384.09 KiB = 826.5 KiB.
+# The kernel's only traffic is its block partials (and the last block's read
+# of them), the tables and the arguments.
+TRAFFIC_BYTES = int((221.25 * 262144.0 / 131092.0 + 384.09375 * 1.0) * 1024)
+OPS_PER_STEP_HESTON32 = DRAWS_PER_HESTON_STEP * (25 + 5) + 14
 def _sync():
     import torch
     torch.cuda.synchronize()
@@ -144,7 +147,7 @@ def main():
     }
     achieved = OPS_PER_STEP_HESTON32 * ps / (k32 * 1e-3) / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TOPS,
-                       "unit": "T lane-cycles/s", "frac": achieved / PEAK_VALU_TOPS,
+                       "unit": "T lane-instr/s", "frac": achieved / PEAK_VALU_TOPS,
                        "traffic": TRAFFIC_BYTES,
                        "kernel": "terminal_feed_kernel<HestonSrc<fp32>, Heston32, CallPayoff>",
                        "ops_per_path_step": OPS_PER_STEP_HESTON32,
@@ -189,6 +192,42 @@ def main():
             lsmc[str(n)] = {"fp32_ms": tl * 1e3, "fp32_price": price, "fp32_std_err": se,
                             "fp64_ms": tl64 * 1e3, "fp64_price": p64, "fp64_std_err": se64}
         out["lsmc_ms"] = lsmc
+
+        # GBM path matrix, 100,000 x 253 FP64 (gbm.rs); kernel time from the
+        # HIP-event probe, wall time including the copy to host memory
+        gkw = dict(s0=100.0, mu=0.05, sigma=0.2, t=1.0, steps=252, n_paths=100_000, seed=SEED)
+        tg = time_calls(lambda: S.gbm(**gkw), 2, 1)
+        kg = kernel_ms(S, lambda: S.gbm(**gkw))
+        gsteps = gkw["n_paths"] * gkw["steps"]
+        out["gbm_paths_100k"] = {"kernel_ms": kg, "wall_ms_incl_d2h": tg * 1e3,
+                                 "path_steps_per_s_kernel": gsteps / (kg * 1e-3),
+                                 "hbm_gbs_kernel": gkw["n_paths"] * 253 * 8 / (kg * 1e-3) / 1e9}
+
+        # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
+        from stocha_amd import batch as B
+        inst = B.config4_instances(4096, seed=SEED)
+        plan = B.Plan(inst, seed=SEED)
+        tb = time_calls(plan.run, 2, 1)
+        out["batch_4096"] = {"ms": tb * 1e3, "path_steps": B.path_steps(inst),
+                             "path_steps_per_s": B.path_steps(inst) / tb, **plan.info()}
+        plan.close()
+
+        # basket call, d = 64 correlated GBM, 262,144 paths x 252 steps, FP32
+        # (L * sum z on MFMA) with the pathwise adjoint for all 64 assets
+        rng = np.random.default_rng(SEED)
+        a = rng.standard_normal((64, 64))
+        c = a @ a.T / 64 + 0.1 * np.eye(64)
+        dg = np.sqrt(np.diag(c))
+        corr = c / np.outer(dg, dg)
+        bs0 = rng.uniform(50, 200, 64)
+        bmu = rng.uniform(0.01, 0.08, 64)
+        bsig = rng.uniform(0.1, 0.4, 64)
+        w = np.full(64, 1.0 / 64)
+        bkw = dict(s0=bs0, mu=bmu, sigma=bsig, corr=corr, weights=w, strike=float(w @ bs0),
+                   r=R, t=T, steps=252, n_paths=262_144, seed=SEED, precision=32, greeks=True)
+        tk = time_calls(lambda: S.basket_price(**bkw), 2, 1)
+        out["basket_d64"] = {"ms": tk * 1e3, "normals_per_s": 262_144 * 252 * 64 / tk,
+                             "price": S.basket_price(**bkw)["price"]}
 
     if args.calib_copy:
         import torch

@@ -174,11 +174,12 @@ int store_paths(int model, const void* mc, uint64_t seed, uint64_t block, uint64
     STOCHA_HIP(hipStreamSynchronize(st));
     return STOCHA_OK;
   }
-  DevBuf d;
-  if (int rc = d.alloc(bytes, err)) return rc;
-  if (int rc = launch_paths(model, mc, sd, n_rows, base_rows, steps, d.as<double>(), st, err))
+  Workspace* ws = nullptr;
+  if (int rc = workspace(bytes, 0, &ws, err)) return rc;
+  if (int rc = launch_paths(model, mc, sd, n_rows, base_rows, steps,
+                            static_cast<double*>(ws->dev), st, err))
     return rc;
-  STOCHA_HIP(hipMemcpyAsync(out, d.p, bytes, hipMemcpyDeviceToHost, st));
+  STOCHA_HIP(hipMemcpyAsync(out, ws->dev, bytes, hipMemcpyDeviceToHost, st));
   STOCHA_HIP(hipStreamSynchronize(st));
   return STOCHA_OK;
 }

@@ -102,17 +102,25 @@ struct MertonSrc {
 // lane computes the output and the next generator state, a lane with a full
 // ring keeps its old state, and the entry is stored unconditionally into the
 // first free slot (one slot is always free, so no live entry is overwritten).
-// Must be called by whole wavefronts (every lane, same steps).
-template <class Src, class Model, uint32_t kSlots, int kCols, class Const>
-__device__ __forceinline__ double queue_path(const Const& c, Pcg& g, const U128 inc,
-                                             uint32_t steps, const ZigLds* zig,
-                                             typename Src::Rec (*q)[kCols]) {
+// hook(step, model) runs after every step (path-storing callers write the
+// value there; it may contain block barriers, since every wave of the block
+// runs the same steps).  Must be called by whole wavefronts (every lane, same
+// steps).  mirror: the model negates its normals (antithetic rows).
+struct NoHook {
+  template <class M>
+  __device__ __forceinline__ void operator()(uint32_t, const M&) const {}
+};
+
+// The loop itself, on a caller-owned model (for models whose state the
+// caller reads afterwards).
+template <class Src, uint32_t kSlots, int kCols, class Model, class Hook = NoHook>
+__device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128 inc,
+                                          uint32_t steps, const ZigLds* zig,
+                                          typename Src::Rec (*q)[kCols], Hook hook = Hook()) {
   static_assert((kSlots & (kSlots - 1)) == 0 && kSlots > Src::kPerStep, "ring size");
   constexpr uint32_t kCap = kSlots - 1;
   constexpr uint32_t kNeed = Src::kPerStep;
   using Rec = typename Src::Rec;
-  Src src(c);
-  Model m(c, false);
   uint32_t head = 0, cnt = 0;
   for (uint32_t step = 0; step < steps; ++step) {
     // bottom-tested: one draw per step is always taken (by lanes with room),
@@ -131,7 +139,18 @@ __device__ __forceinline__ double queue_path(const Const& c, Pcg& g, const U128 
     Src::template apply<kSlots, kCols>(m, q, head);
     head = (head + kNeed) & (kSlots - 1);
     cnt -= kNeed;
+    hook(step, m);
   }
+}
+
+template <class Src, class Model, uint32_t kSlots, int kCols, class Const, class Hook = NoHook>
+__device__ __forceinline__ double queue_path(const Const& c, Pcg& g, const U128 inc,
+                                             uint32_t steps, const ZigLds* zig,
+                                             typename Src::Rec (*q)[kCols], bool mirror = false,
+                                             Hook hook = Hook()) {
+  Src src(c);
+  Model m(c, mirror);
+  queue_run<Src, kSlots, kCols>(src, m, g, inc, steps, zig, q, hook);
   return m.terminal();
 }
 

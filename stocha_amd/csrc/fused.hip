@@ -197,29 +197,28 @@ __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
   finish_sums(acc, pay.n_acc(), red, partials, counter, out);
 }
 
-// Sums partials[b][i] over b in block order; one thread per accumulator
-// slot strides over blocks, then a fixed tree over the 256 threads.
+// Sums partials[b][i] over b: block i owns accumulator i; thread t adds
+// blocks t, t+256, ... in order, then a fixed tree over the 256 threads, so
+// the result does not depend on timing.
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ partials,
                                                               const int n_blocks, const int na,
                                                               double* __restrict__ out) {
   __shared__ double buf[256];
-  for (int i = 0; i < na; ++i) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < n_blocks; b += 256) s += partials[(uint64_t)b * na + i];
-    buf[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if (threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[i] = buf[0];
+  const int i = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < n_blocks; b += 256) s += partials[(uint64_t)b * na + i];
+  buf[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[i] = buf[0];
 }
 
 int launch_reduce_partials(const double* part, int n_blocks, int na, double* out, hipStream_t st,
                            const Err& err) {
-  reduce_partials_kernel<<<1, 256, 0, st>>>(part, n_blocks, na, out);
+  reduce_partials_kernel<<<na, 256, 0, st>>>(part, n_blocks, na, out);
   STOCHA_HIP(hipGetLastError());
   return STOCHA_OK;
 }

@@ -19,6 +19,7 @@
 #include <stdlib.h>
 
 #include "host.hpp"
+#include "drivers.hpp"
 
 namespace stocha {
 
@@ -52,37 +53,56 @@ __device__ __forceinline__ double powi_rt(double a, int b) {
 }
 
 // ------------------------------------------------------------ generation --
+// GBM paths on the queue schedule (drivers.hpp); the step hook writes S_{j+1}
+// of path i to P[j * n + i] (coalesced across the wave).  Lanes past n step
+// path 0's stream and write nothing.  FP32: S = s0 * 2^(ls * log2 e) in FP32.
+template <bool kFp32>
+struct DateMajorHook {
+  void* paths;
+  uint64_t n, i;
+  bool valid;
+  float s0f;
+  template <class M>
+  __device__ __forceinline__ void operator()(uint32_t j, const M& m) const {
+    if (!valid) return;
+    if constexpr (kFp32)
+      ((float*)paths)[(uint64_t)j * n + i] = s0f * __builtin_amdgcn_exp2f(m.ls * 1.44269504088896341f);
+    else
+      ((double*)paths)[(uint64_t)j * n + i] = m.s;
+  }
+};
+
 template <bool kFp32>
 __global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, const StreamDesc sd,
-                                                            const uint64_t n, const uint64_t steps,
+                                                            const uint64_t n, const uint32_t steps,
                                                             void* __restrict__ paths,
                                                             double* __restrict__ cf, const double k,
                                                             const int is_put) {
+  using Src = GbmSrc<kFp32>;
+  using Model = typename std::conditional<kFp32, Gbm32, Gbm64>::type;
   __shared__ ZigLds zig;
+  __shared__ typename Src::Rec q[8][kLsBlock];
   zig_load(&zig);
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * kLsBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kLsBlock + threadIdx.x; i < n; i += stride) {
-    Pcg g{stream_at(&sd, i)};
-    const U128 inc = sd.inc;
-    double last;
-    if constexpr (kFp32) {
-      Gbm32 m(mc, false);
-      float* P = (float*)paths;
-      float s = 0.f;
-      for (uint64_t j = 0; j < steps; ++j) {
-        const float ls = m.step(g, inc, &zig);
-        s = (float)mc.s0 * __builtin_amdgcn_exp2f(ls * 1.44269504088896341f);
-        P[j * n + i] = s;
-      }
-      last = s;
-    } else {
-      Gbm64 m(mc, false);
-      double* P = (double*)paths;
-      for (uint64_t j = 0; j < steps; ++j) P[j * n + i] = m.step(g, inc, &zig);
-      last = m.s;
+  for (uint64_t base = (uint64_t)blockIdx.x * kLsBlock; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    DateMajorHook<kFp32> hook;
+    hook.paths = paths;
+    hook.n = n;
+    hook.i = i;
+    hook.valid = i < n;
+    hook.s0f = (float)mc.s0;
+    Pcg g{stream_at(&sd, hook.valid ? i : 0)};
+    (void)queue_path<Src, Model, 8, kLsBlock>(mc, g, sd.inc, steps, &zig, q, false, hook);
+    if (hook.valid) {
+      double last;
+      if constexpr (kFp32)
+        last = (double)((const float*)paths)[(uint64_t)(steps - 1) * n + i];
+      else
+        last = ((const double*)paths)[(uint64_t)(steps - 1) * n + i];
+      cf[i] = intrinsic(last, k, is_put);  // lsmc.rs:45-47
     }
-    cf[i] = intrinsic(last, k, is_put);  // lsmc.rs:45-47
   }
 }
 
@@ -636,10 +656,11 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   // paths per thread the spots of date+1 are read again (the previous date's
   // column, still in the caches).  Registers stay free for the regression.
   constexpr bool kKeepSpot = kPpt <= 8;
+  constexpr int kUnrollQ = kPpt <= 8 ? kPpt : 4;  // per-path loops: full unroll up to 8
   __shared__ double cf[kPpt][kCoopBlock];
   __shared__ double sp[kKeepSpot ? kPpt : 1][kCoopBlock];
   const int tx = threadIdx.x;
-#pragma unroll(kPpt <= 8 ? kPpt : 4)
+#pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
     const uint64_t i = t0 + q * G;
     const double s0 = i < a.n ? spot_at<kFp32>(a.paths, a.n, a.steps, i) : 0.0;
@@ -666,7 +687,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
 #pragma unroll
     for (int e = 0; e < kMaxAccB; ++e) acc[e] = 0.0;
     const double inv_sc = 1.0 / sc;
-#pragma unroll(kPpt <= 8 ? kPpt : 4)
+#pragma unroll kUnrollQ
     for (int q = 0; q < kPpt; ++q) {
       const uint64_t i = t0 + q * G;
       if (i < a.n) {
@@ -742,7 +763,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   }
   // exercise at date 1, final discount (lsmc.rs:102-104), sum
   double acc1[kMaxAccB] = {};
-#pragma unroll(kPpt <= 8 ? kPpt : 4)
+#pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
     const uint64_t i = t0 + q * G;
     if (i < a.n) {
@@ -762,7 +783,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   coop_total(a.part, 1, red, tot);
   const double mean = tot[0] / (double)a.n;
   double acc2[kMaxAccB] = {};
-#pragma unroll(kPpt <= 8 ? kPpt : 4)
+#pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
     const uint64_t i = t0 + q * G;
     if (i < a.n) {
@@ -816,6 +837,7 @@ template <bool kFp32>
 static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint64_t steps,
                      double k, double discount, int is_put, int ncols, double* price,
                      double* std_err, hipStream_t st, const Err& err) {
+  if (steps > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps must be below 2^32");
   const size_t esz = kFp32 ? sizeof(float) : sizeof(double);
   uint64_t nb64 = (n + kLsBlock - 1) / kLsBlock;
   const int nb = (int)(nb64 > 2048 ? 2048 : nb64);
@@ -842,7 +864,8 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
 
   KernelTimer timer(st);
   timer.start();
-  lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, steps, paths, cf, k, is_put);
+  lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, (uint32_t)steps, paths, cf, k,
+                                                  is_put);
   STOCHA_HIP(hipGetLastError());
   if (coop) {
     LsmcCoopArgs ca;

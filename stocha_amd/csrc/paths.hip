@@ -2,51 +2,31 @@
 // merton_jump_diffusion outputs).
 //
 // One thread owns one output row (one path, or its antithetic mirror) and
-// steps it in registers with the reference's arithmetic (models.hpp).  The
-// output is the reference's row-major (n_rows, steps+1) matrix, so a thread's
-// consecutive values are 8 bytes apart and neighbouring threads are a whole
-// row apart.  To make the HBM writes coalesced the block stages kTile columns
-// of its 256 rows in LDS and then writes them row by row: 16 consecutive
-// lanes cover one 128-byte row segment.
+// steps it with the reference's arithmetic (models.hpp) on the queue schedule
+// (drivers.hpp: per-lane ziggurat state machine, steps applied by the whole
+// wave).  The output is the reference's row-major (n_rows, steps+1) matrix,
+// so a thread's consecutive values are 8 bytes apart and neighbouring threads
+// are a whole row apart.  To make the HBM writes coalesced the block stages
+// kTile columns of its 256 rows in LDS and then writes them row by row: 8
+// consecutive lanes cover one 64-byte row segment.
 #include "host.hpp"
+#include "drivers.hpp"
 
 namespace stocha {
 
 constexpr int kPathBlock = 256;
-constexpr int kTile = 16;
+constexpr int kTile = 8;
 
-template <class Model, class Const>
-__global__ __launch_bounds__(kPathBlock) void paths_kernel(const Const mc, const StreamDesc sd,
-                                                           const uint64_t n_rows,
-                                                           const uint64_t base_rows,
-                                                           const uint64_t steps,
-                                                           double* __restrict__ out) {
-  __shared__ ZigLds zig;
-  __shared__ double tile[kTile][kPathBlock + 1];
-  zig_load(&zig);
-
-  const uint64_t row0 = (uint64_t)blockIdx.x * kPathBlock;
-  const uint64_t row = row0 + threadIdx.x;
-  // Threads past the last row still take part in the barriers; they step a
-  // valid stream (path 0) and their values are never written.
-  const bool valid = row < n_rows;
-  const bool mirror = valid && row >= base_rows;
-  const uint64_t path = !valid ? 0 : (mirror ? row - base_rows : row);
-  Pcg g{stream_at(&sd, path)};
-  const U128 inc = sd.inc;
-  Model m(mc, mirror);
-  __syncthreads();
-
-  const uint64_t width = steps + 1;
-  const uint64_t rows_here = n_rows - row0 < (uint64_t)kPathBlock ? n_rows - row0 : kPathBlock;
-  for (uint64_t c0 = 0; c0 < width; c0 += kTile) {
-    const int cols = (int)((width - c0) < (uint64_t)kTile ? (width - c0) : kTile);
-    for (int j = 0; j < cols; ++j) {
-      const uint64_t col = c0 + j;
-      tile[j][threadIdx.x] = col == 0 ? mc.s0 : m.step(g, inc, &zig);
-    }
+// Step hook: column step+1 of the block's rows goes to the LDS tile; a full
+// tile (or the last column) is written out row-major.  Runs on every thread
+// of the block after each step, so its barriers line up.
+struct TileHook {
+  double (*tile)[kPathBlock + 1];
+  double* out;
+  uint64_t row0, width, rows_here;
+  __device__ __forceinline__ void flush(uint64_t c0, int cols) const {
     __syncthreads();
-    // Row-major write-out: element e of the tile is (row e / cols, col e % cols).
+    // element e of the tile is (row e / cols, col e % cols)
     const int n_el = (int)rows_here * cols;
     for (int e = threadIdx.x; e < n_el; e += kPathBlock) {
       const int r = e / cols;
@@ -55,6 +35,45 @@ __global__ __launch_bounds__(kPathBlock) void paths_kernel(const Const mc, const
     }
     __syncthreads();
   }
+  template <class M>
+  __device__ __forceinline__ void operator()(uint32_t step, const M& m) const {
+    const uint64_t col = (uint64_t)step + 1;
+    const int j = (int)(col % kTile);
+    tile[j][threadIdx.x] = m.terminal();
+    if (j == kTile - 1 || col == width - 1) flush(col - j, j + 1);
+  }
+};
+
+template <class Src, class Model, class Const, uint32_t kSlots>
+__global__ __launch_bounds__(kPathBlock) void paths_kernel(const Const mc, const StreamDesc sd,
+                                                           const uint64_t n_rows,
+                                                           const uint64_t base_rows,
+                                                           const uint32_t steps,
+                                                           double* __restrict__ out) {
+  using Rec = typename Src::Rec;
+  __shared__ ZigLds zig;
+  __shared__ Rec q[kSlots][kPathBlock];
+  __shared__ double tile[kTile][kPathBlock + 1];
+  zig_load(&zig);
+
+  const uint64_t row0 = (uint64_t)blockIdx.x * kPathBlock;
+  const uint64_t row = row0 + threadIdx.x;
+  // Threads past the last row still take part in the steps and barriers;
+  // they step a valid stream (path 0) and their values are never written.
+  const bool valid = row < n_rows;
+  const bool mirror = valid && row >= base_rows;
+  const uint64_t path = !valid ? 0 : (mirror ? row - base_rows : row);
+  Pcg g{stream_at(&sd, path)};
+  tile[0][threadIdx.x] = mc.s0;
+  __syncthreads();
+
+  TileHook hook;
+  hook.tile = tile;
+  hook.out = out;
+  hook.row0 = row0;
+  hook.width = (uint64_t)steps + 1;
+  hook.rows_here = n_rows - row0 < (uint64_t)kPathBlock ? n_rows - row0 : kPathBlock;
+  (void)queue_path<Src, Model, kSlots, kPathBlock>(mc, g, sd.inc, steps, &zig, q, mirror, hook);
 }
 
 int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n_rows,
@@ -65,18 +84,20 @@ int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n
   dim3 grid((unsigned)blocks), block(kPathBlock);
   KernelTimer timer(st);
   timer.start();
+  if (steps > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps must be below 2^32");
+  const uint32_t st32 = (uint32_t)steps;
   switch (model) {
     case kGbm:
-      paths_kernel<Gbm64, GbmConst><<<grid, block, 0, st>>>(*(const GbmConst*)mconst, sd, n_rows,
-                                                            base_rows, steps, d_out);
+      paths_kernel<GbmSrc<false>, Gbm64, GbmConst, 8><<<grid, block, 0, st>>>(
+          *(const GbmConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     case kHeston:
-      paths_kernel<Heston64, HestonConst><<<grid, block, 0, st>>>(
-          *(const HestonConst*)mconst, sd, n_rows, base_rows, steps, d_out);
+      paths_kernel<HestonSrc<false>, Heston64, HestonConst, 8><<<grid, block, 0, st>>>(
+          *(const HestonConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     case kMerton:
-      paths_kernel<Merton64, MertonConst><<<grid, block, 0, st>>>(
-          *(const MertonConst*)mconst, sd, n_rows, base_rows, steps, d_out);
+      paths_kernel<MertonSrc<false, Merton64>, Merton64, MertonConst, 4><<<grid, block, 0, st>>>(
+          *(const MertonConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     default:
       return err.set(STOCHA_EINVAL, "unknown model id %d", model);
