@@ -10,9 +10,11 @@
 //     W = sum over steps of z_t,
 // so a path keeps the running sum W (one LDS column per lane, d entries) and
 // forms E = L W once at the end (VALU dot products with L read through the
-// scalar cache; or, in the FP32 form, v_mfma_f32_16x16x4f32 tiles over the
+// scalar cache; or, in the FP32 form, v_mfma_f32_16x16x4f32 tiles over a
 // wave's 64 paths).  This equals the reference's per-step L z_t products in
-// exact arithmetic; rounding differs by the reassociation only.
+// exact arithmetic; rounding differs by the reassociation only.  The FP32
+// form runs in two kernels (the draws, then payoff and adjoint from W kept
+// asset-major in the workspace); the FP64 form in one.
 //
 // Adjoint of one path (reverse mode through the payoff):
 //   B = sum_a w_a S_a,  Bbar = 1{B > K},  Sbar_a = w_a Bbar

@@ -12,8 +12,9 @@
 // indices from a device counter.  Each tile's payoff sum and sum of squares
 // go to its own slot, and a second kernel adds every instance's tile slots in
 // tile order, so results do not depend on which block ran which tile.
-// Within a tile every lane runs the per-lane draw-feed state machine
-// (zigsm.hpp) on its own path; the model switch is uniform over the block.
+// Within a tile every lane runs its own path on the queue schedule
+// (drivers.hpp: per-lane ziggurat state machine, steps applied by the whole
+// wave); the model switch is uniform over the block.
 //
 // The three launches (counter reset, persistent kernel, per-instance
 // reduction) are captured once in a hipGraph and replayed; the counter is
