@@ -240,11 +240,11 @@ static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths,
   const unsigned grid = fused_grid(n_paths);
   Workspace* ws = nullptr;
   const size_t part_bytes = sizeof(double) * grid * na;
-  if (int rc = workspace(part_bytes + sizeof(double) * kMaxAcc, sizeof(double) * kMaxAcc, &ws,
-                         err))
-    return rc;
+  if (int rc = workspace(part_bytes, sizeof(double) * kMaxAcc, &ws, err)) return rc;
   double* part = static_cast<double*>(ws->dev);
-  double* out = part + (size_t)grid * na;
+  // the last block writes the totals straight into the pinned host buffer
+  // (device-visible through unified addressing; read after the stream sync)
+  double* out = ws->host;
   KernelTimer timer(st);
   timer.start();
   if (schedule == kScheduleLoop)
@@ -255,7 +255,6 @@ static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths,
         mc, sd, n_paths, (uint32_t)steps, pay, part, ws->counter, out);
   STOCHA_HIP(hipGetLastError());
   timer.stop(1);
-  STOCHA_HIP(hipMemcpyAsync(ws->host, out, sizeof(double) * na, hipMemcpyDeviceToHost, st));
   STOCHA_HIP(hipStreamSynchronize(st));
   for (int i = 0; i < na; ++i) h_out[i] = ws->host[i];
   return STOCHA_OK;

@@ -202,40 +202,49 @@ __device__ __forceinline__ void zig_load(ZigLds* z) {
   }
 }
 
+// next_f64 of a draw (pcg64.rs:37-39) rounded to FP32, from its bits: the
+// top 32 and the next 21 bits of m = u >> 11, so the result is within
+// 2^-24 relative of m * 2^-53 for every m > 0 (also tiny m).
+__device__ __forceinline__ float u53_to_f32(uint64_t u) {
+  uint32_t lo = ((uint32_t)u) >> 11;
+  asm("" : "+v"(lo));
+  return fmaf((float)lo, 0x1p-53f, hi32_to_f32(u) * 0x1p-32f);
+}
+
 // normal.rs:54-67, one (u1, u2) pair from draws a, b: returns whether the pair
 // is accepted and then the tail value x + R (x = -ln(u1) / R) in *t.
 // The accept test -2 ln(u2) >= x^2 is evaluated with the hardware log2 in
 // FP32 and decided there unless |d| = |-2 ln u2 - x^2| is inside a guard band
 // of 1e-5 (1 + |terms|); there the FP64 test is recomputed (the FP32 error is
-// below 2e-6 (1 + |terms|): FP32 rounding of u, log2 within a few ulp), so
-// the decision -- and the draws consumed -- are always the FP64 ones.
-__device__ __forceinline__ bool zig_tail_test(double u1, double u2) {
-  const float l1 = __builtin_amdgcn_logf((float)u1);  // log2, u >= 2^-53 is a normal float
-  const float l2 = __builtin_amdgcn_logf((float)u2);
+// below 2e-6 (1 + |terms|): u rounded to FP32, log2 within a few ulp), so the
+// decision -- and the draws consumed -- are always the FP64 ones.  A zero
+// uniform rejects the pair (normal.rs:59-61).
+__device__ __forceinline__ bool zig_tail_test(uint64_t a, uint64_t b, float* l1_out) {
+  const float l1 = __builtin_amdgcn_logf(u53_to_f32(a));  // log2; u >= 2^-53 is normal
+  const float l2 = __builtin_amdgcn_logf(u53_to_f32(b));
+  *l1_out = l1;
   const float x = l1 * (float)(-0.69314718055994531 / kZigR);
-  const float a = l2 * -1.3862943611198906f;
+  const float aa = l2 * -1.3862943611198906f;
   const float x2 = x * x;
-  const float d = a - x2;
-  if (fabsf(d) > 1e-5f * (1.f + a + x2)) return d >= 0.f;
+  const float d = aa - x2;
+  if (fabsf(d) > 1e-5f * (1.f + aa + x2)) return d >= 0.f;
+  const double u1 = (double)(a >> 11) * 0x1p-53, u2 = (double)(b >> 11) * 0x1p-53;
   const double xd = -log(u1) / kZigR;
   return -2.0 * log(u2) >= xd * xd;
 }
 __device__ __forceinline__ bool zig_tail_pair64(uint64_t a, uint64_t b, double* t) {
-  const double u1 = (double)(a >> 11) * 0x1p-53;  // next_f64 (pcg64.rs:37-39)
-  const double u2 = (double)(b >> 11) * 0x1p-53;
-  if (u1 <= 0.0 || u2 <= 0.0) return false;
-  if (!zig_tail_test(u1, u2)) return false;
-  *t = -log(u1) / kZigR + kZigR;
+  if ((a >> 11) == 0 || (b >> 11) == 0) return false;
+  float l1;
+  if (!zig_tail_test(a, b, &l1)) return false;
+  *t = -log((double)(a >> 11) * 0x1p-53) / kZigR + kZigR;
   return true;
 }
-// FP32 form: same decision; the value from the FP32 log (within ~1 ulp of
-// the FP64 value rounded to FP32).
+// FP32 form: same decision; the value from the FP32 log (within ~2 ulp of the
+// FP64 value rounded to FP32).
 __device__ __forceinline__ bool zig_tail_pair32(uint64_t a, uint64_t b, float* t) {
-  const double u1 = (double)(a >> 11) * 0x1p-53;
-  const double u2 = (double)(b >> 11) * 0x1p-53;
-  if (u1 <= 0.0 || u2 <= 0.0) return false;
-  if (!zig_tail_test(u1, u2)) return false;
-  const float l1 = __builtin_amdgcn_logf((float)u1);
+  if ((a >> 11) == 0 || (b >> 11) == 0) return false;
+  float l1;
+  if (!zig_tail_test(a, b, &l1)) return false;
   *t = fmaf(l1, (float)(-0.69314718055994531 / kZigR), (float)kZigR);
   return true;
 }

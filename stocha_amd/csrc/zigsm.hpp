@@ -32,75 +32,78 @@ namespace stocha {
 template <bool kFp32>
 struct ZigFeed {
   using Real = typename std::conditional<kFp32, float, double>::type;
-  uint32_t state = 0;
-  uint32_t idx = 0;  // table index (layer | sign << 8) of a pending wedge / tail
-  uint64_t ua = 0;   // wedge: the attempt draw (x); tail: the pair's first draw
+  uint32_t state = 0;  // 0 attempt, 1 wedge uniform next, 2 / 3 tail pair
+  uint32_t idx = 0;    // table index (layer | sign << 8) of a pending wedge / tail
+  uint64_t ua = 0;     // wedge: the attempt draw; tail: the pair's first draw
+  Real xw = 0;         // wedge: the attempt's (signed) x
 
-  // Consumes draw u when `active` (state unchanged otherwise).  The common
-  // case is straight-line on every lane: one table read, a 32-bit test, the
-  // value; lanes not in the attempt state, or whose attempt is not a clear
-  // fast accept, take the one divergent block (feed_slow).  `out` is written
-  // on every call and holds a normal only when the call returns true.
+  // Consumes draw u when `active` (state unchanged otherwise).  Straight-line
+  // on every lane: the table read, the fast test and value, and the cheap
+  // transitions as selects (a missed attempt records its draw and x; the
+  // first draw of a tail pair is recorded).  Only lanes whose draw must be
+  // evaluated -- a wedge uniform or the second draw of a tail pair -- take
+  // the divergent block (feed_eval).  `out` is written on every call and
+  // holds a normal only when the call returns true.
   __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* __restrict__ z, Real& out,
                                        bool active = true) {
     const uint32_t i = (uint32_t)u & 0x1ffu;
     const uint32_t hi = (uint32_t)(u >> 32);
-    bool fast;
+    uint32_t th;
     if constexpr (kFp32) {
       const ZigFast32 f = z->fast32[i];
-      fast = hi < f.th;
+      th = f.th;
       out = hi32_to_f32(u) * f.xs;
     } else {
       const ZigFast f = z->fast[i];
-      fast = hi < f.th;
+      th = f.th;
       out = ((double)(u >> 11) * 0x1p-53) * f.x;
     }
-    const bool take = active && state == 0 && fast;
-    if (active && !take) return feed_slow(u, z, out);
+    // (bitwise on bools: no short-circuit control flow)
+    const bool at = active & (state == 0);
+    const bool take = at & (hi < th);
+    // a tie on the high word (one draw in 2^32) is booked as a miss here and
+    // settled with the full threshold in feed_eval
+    const bool tie = at & (hi == th);
+    const bool slow = (active & ((state == 1u) | (state == 3u))) | tie;
+    const bool miss = at & !take;
+    const bool tail1 = active & (state == 2);
+    ua = (miss | tail1) ? u : ua;
+    idx = miss ? i : idx;
+    xw = miss ? out : xw;
+    state = miss ? ((i & 0xffu) == 0 ? 2u : 1u) : (tail1 ? 3u : state);
+    if (slow) return feed_eval(u, z, out, tie);
     return take;
   }
 
-  __device__ __forceinline__ bool feed_slow(uint64_t u, const ZigLds* __restrict__ z, Real& out) {
-    if (state == 0) {  // attempt that missed the 32-bit test
-      const uint32_t i = (uint32_t)u & 0x1ffu, l = i & 0xffu;
-      if ((u >> 11) < z->t[l]) return true;  // high-word tie, fast accept after all; out holds x
-      ua = u;
-      idx = i;
-      state = l == 0 ? 2u : 1u;
-      return false;
+  __device__ __forceinline__ bool feed_eval(uint64_t u, const ZigLds* __restrict__ z, Real& out,
+                                            bool tie) {
+    if (tie) {  // exact fast-path test (normal.rs:26-29); out holds x
+      if ((u >> 11) < z->t[idx & 0xffu]) {
+        state = 0;
+        return true;
+      }
+      return false;  // a real miss, already booked
     }
     if (state == 1) {  // wedge (normal.rs:36-42): u is the uniform
       state = 0;
       const uint32_t l = idx & 0xffu;
+      out = xw;
       if constexpr (kFp32) {
         const float2 yd = z->ydy32[l];
         const float y = fmaf(top24_to_f32(u) * 0x1p-24f, yd.y, yd.x);
-        const float x = hi32_to_f32(ua) * z->fast32[idx].xs;
-        const float e = __builtin_amdgcn_exp2f(-0.72134752044448170f * x * x);
+        const float e = __builtin_amdgcn_exp2f(-0.72134752044448170f * xw * xw);
         const float diff = y - e;
-        bool accept;
-        if (fabsf(diff) > 4e-6f) {
-          accept = diff < 0.f;
-        } else {
-          const double x64 = ((double)(ua >> 11) * 0x1p-53) * z->x[l];
-          const double y64 = z->y[l] + ((double)(u >> 11) * 0x1p-53) * (z->y[l + 1] - z->y[l]);
-          accept = y64 < exp(-0.5 * x64 * x64);
-        }
-        out = x;
-        return accept;
+        if (fabsf(diff) > 4e-6f) return diff < 0.f;
+        const double x64 = ((double)(ua >> 11) * 0x1p-53) * z->x[l];
+        const double y64 = z->y[l] + ((double)(u >> 11) * 0x1p-53) * (z->y[l + 1] - z->y[l]);
+        return y64 < exp(-0.5 * x64 * x64);
       } else {
-        const double x = ((double)(ua >> 11) * 0x1p-53) * z->fast[idx].x;
-        const double y = z->y[l] + ((double)(u >> 11) * 0x1p-53) * (z->y[l + 1] - z->y[l]);
-        out = x;
-        return y < exp(-0.5 * x * x);
+        const double y0 = z->y[l], y1 = z->y[l + 1];
+        const double y = y0 + ((double)(u >> 11) * 0x1p-53) * (y1 - y0);
+        return y < exp(-0.5 * xw * xw);
       }
     }
-    if (state == 2) {  // tail (normal.rs:54-67): the pair's first draw
-      ua = u;
-      state = 3;
-      return false;
-    }
-    state = 2;  // the pair's second draw
+    state = 2;  // tail (normal.rs:54-67): the pair's second draw
     Real t;
     bool ok;
     if constexpr (kFp32) {
