@@ -624,11 +624,14 @@ __device__ __forceinline__ void coop_total(const double* part, int na,
 
 // Exercise decision of lsmc.rs:91-98 with the coefficients of the last
 // regression date; returns the (undiscounted) cash flow.
+// (x is formed with the reciprocal of the std, one multiply instead of a
+// division per path; the regression coefficients already differ from the
+// reference's QR in the last bits, so this changes nothing measurable.)
 __device__ __forceinline__ double exercise(double cf, double s, double k, int is_put, int ncols,
-                                           double mean, double sdv, const double* c) {
+                                           double mean, double inv_sdv, const double* c) {
   const double exv = intrinsic(s, k, is_put);
   if (exv > 0.0) {
-    const double x = (s - mean) / sdv;
+    const double x = (s - mean) * inv_sdv;
     double cont = 0.0;
 #pragma unroll
     for (int j = 0; j < kMaxCols; ++j)
@@ -668,7 +671,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     cf[q][tx] = i < a.n ? intrinsic(s0, a.k, a.is_put) : 0.0;  // lsmc.rs:45-47
   }
   bool have_ex = false;  // a regression was solved at date+1
-  double ex_mean = 0.0, ex_std = 1.0, ex_c[kMaxCols];
+  double ex_mean = 0.0, ex_std = 1.0, ex_inv = 1.0, ex_c[kMaxCols];
   for (int j = 0; j < kMaxCols; ++j) ex_c[j] = 0.0;
   double c = a.k, sc = a.k;  // moment variable u = (S - c) / sc
   // Up to 4 paths per thread the spots of the next date are loaded at the end
@@ -695,7 +698,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         if (have_ex) {
           const double s1 =
               kKeepSpot ? sp[kKeepSpot ? q : 0][tx] : spot_at<kFp32>(a.paths, a.n, date + 1, i);
-          v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_std, ex_c);
+          v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
         }
         const double y = v * a.discount;  // lsmc.rs:52-54
         cf[q][tx] = y;
@@ -754,6 +757,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     if (have_ex) {
       ex_mean = ms_s[0];
       ex_std = ms_s[1];
+      ex_inv = 1.0 / ex_std;
 #pragma unroll
       for (int j = 0; j < kMaxCols; ++j) ex_c[j] = coef_s[j];
       c = ex_mean;
@@ -770,7 +774,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
       double v = cf[q][tx];
       if (have_ex) {
         const double s1 = kKeepSpot ? sp[kKeepSpot ? q : 0][tx] : spot_at<kFp32>(a.paths, a.n, 1, i);
-        v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_std, ex_c);
+        v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
       }
       cf[q][tx] = v * a.discount;
       acc1[0] += cf[q][tx];
