@@ -121,6 +121,17 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
   constexpr uint32_t kCap = kSlots - 1;
   constexpr uint32_t kNeed = Src::kPerStep;
   using Rec = typename Src::Rec;
+  // The two increment words that meet a carry-in (v_addc) live in VGPRs for
+  // the whole loop: an SGPR operand next to VCC would exceed gfx9's one
+  // scalar operand per VALU instruction and cost a v_mov per draw.
+  U128 incv = inc;
+  {
+    uint32_t w1 = (uint32_t)(inc.lo >> 32), w2 = (uint32_t)inc.hi;
+    asm("" : "+v"(w1));
+    asm("" : "+v"(w2));
+    incv.lo = ((uint64_t)w1 << 32) | (uint32_t)inc.lo;
+    incv.hi = (inc.hi & 0xffffffff00000000ull) | w2;
+  }
   uint32_t head = 0, cnt = 0;
   for (uint32_t step = 0; step < steps; ++step) {
     // bottom-tested: one draw per step is always taken (by lanes with room),
@@ -128,7 +139,7 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
     do {
       const bool active = cnt < kCap;
       const uint64_t u = g.output();
-      const U128 sn = g.stepped(inc);
+      const U128 sn = g.stepped(incv);
       g.s.lo = active ? sn.lo : g.s.lo;
       g.s.hi = active ? sn.hi : g.s.hi;
       Rec r;

@@ -90,9 +90,9 @@ struct Pcg {
     const uint64_t a = (uint64_t)h0 * m0 + x;  // wraps mod 2^64, as wanted
     const uint32_t cross = h0 * m1 + h1 * m0;
     uint32_t c0, c1, c2;
-    const uint32_t n0 = __builtin_addc((uint32_t)t, (uint32_t)inc.lo, 0u, &c0);
-    const uint32_t n1 = __builtin_addc((uint32_t)w, (uint32_t)(inc.lo >> 32), c0, &c1);
-    const uint32_t n2 = __builtin_addc((uint32_t)a, (uint32_t)inc.hi, c1, &c2);
+    const uint32_t n0 = __builtin_addc((uint32_t)inc.lo, (uint32_t)t, 0u, &c0);
+    const uint32_t n1 = __builtin_addc((uint32_t)(inc.lo >> 32), (uint32_t)w, c0, &c1);
+    const uint32_t n2 = __builtin_addc((uint32_t)inc.hi, (uint32_t)a, c1, &c2);
     const uint32_t n3 = (uint32_t)(a >> 32) + cross + (uint32_t)(inc.hi >> 32) + c2;
     U128 n;
     n.lo = ((uint64_t)n1 << 32) | n0;
@@ -132,9 +132,9 @@ __device__ __forceinline__ U128 stream_at(const StreamDesc* __restrict__ d, uint
 // makes the 32-bit operand opaque so a single v_cvt_f32_u32 is emitted.  (No
 // memory access is involved.)
 __device__ __forceinline__ float hi32_to_f32(uint64_t u) {
-  uint32_t h = (uint32_t)(u >> 32);
-  asm("" : "+v"(h));
-  return (float)h;
+  float f;
+  asm("v_cvt_f32_u32 %0, %1" : "=v"(f) : "v"((uint32_t)(u >> 32)));
+  return f;
 }
 __device__ __forceinline__ float top24_to_f32(uint64_t u) {  // (u >> 40) as float, exact
   uint32_t h = (uint32_t)(u >> 40);
