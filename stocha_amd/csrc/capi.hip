@@ -104,6 +104,8 @@ MertonConst merton_const(const double* p, double t, uint64_t steps) {
   c.drift = (mu - 0.5 * sigma * sigma - lambda * m_bar) * dt;  // :56
   c.diffusion = sigma * sqrt_dt;                               // :57
   c.jump_prob = lambda * dt;                                   // :60
+  const double scaled = ldexp(c.jump_prob, 53);
+  c.jump_t53 = !(scaled > 0.0) ? 0 : scaled >= 0x1p53 ? (1ull << 53) : (uint64_t)ceil(scaled);
   c.mu_j = mu_j;
   c.sigma_j = sigma_j;
   c.drift_f = (float)c.drift;

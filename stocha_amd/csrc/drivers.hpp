@@ -58,33 +58,24 @@ struct MertonSrc {
   ZigFeed<kFp32> zf;
   uint32_t phase = 0;  // 0: z next, 1: uniform next, 2: jump normal next
   Real zz = 0;
-  double p;  // lambda * dt
-  M jm;      // for jump_return()
+  uint64_t t53;  // next_f64() < lambda*dt  <=>  (u >> 11) < t53
+  M jm;          // for jump_return()
   template <class C>
-  __device__ __forceinline__ explicit MertonSrc(const C& c) : p(c.jump_prob), jm(c, false) {}
+  __device__ __forceinline__ explicit MertonSrc(const C& c) : t53(c.jump_t53), jm(c, false) {}
+  // Straight-line: the phase transitions and the entry are selects; only the
+  // ziggurat's own slow block branches.  rec is written on every call.
   __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* z, Rec& r, bool active) {
     Real x;
-    const bool uni = phase == 1;
-    const bool got = zf.feed(u, z, x, active && !uni);
-    const bool jump = (double)(u >> 11) * 0x1p-53 < p;  // next_f64 (pcg64.rs:37-39)
-    bool out = false;
-    if (active && uni) {
-      phase = jump ? 2u : 0u;
-      r.z = zz;
-      r.jr = (Real)0;
-      out = !jump;
-    } else if (got) {
-      if (phase == 0) {
-        zz = x;
-        phase = 1;
-      } else {
-        phase = 0;
-        r.z = zz;
-        r.jr = jm.jump_return(x);
-        out = true;
-      }
-    }
-    return out;
+    const bool uni = active & (phase == 1u);
+    const bool got = zf.feed(u, z, x, active & !uni);
+    const bool jump = (u >> 11) < t53;  // the uniform draw (pcg64.rs:37-39)
+    const bool got0 = got & (phase == 0u);
+    const bool got2 = got & (phase == 2u);
+    r.z = zz;
+    r.jr = got2 ? jm.jump_return(x) : (Real)0;
+    zz = got0 ? x : zz;
+    phase = uni ? (jump ? 2u : 0u) : (got0 ? 1u : (got2 ? 0u : phase));
+    return (uni & !jump) | got2;
   }
   template <uint32_t kSlots, int kCols, class MM>
   __device__ static __forceinline__ void apply(MM& m, const Rec (*q)[kCols], uint32_t head) {

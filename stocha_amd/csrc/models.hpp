@@ -40,6 +40,10 @@ struct HestonConst {
 
 struct MertonConst {
   double s0, drift, diffusion, jump_prob, mu_j, sigma_j;
+  // next_f64() < jump_prob  <=>  (u >> 11) < jump_t53: next_f64 is m * 2^-53
+  // exactly, so the comparison is m < jump_prob * 2^53 (an exact scaling),
+  // i.e. m < ceil(jump_prob * 2^53), clamped to [0, 2^53].
+  uint64_t jump_t53;
   float drift_f, diffusion_f, mu_j_f, sigma_j_f;
 };
 
@@ -101,8 +105,7 @@ struct Merton64 {
   }
   __device__ __forceinline__ double step(Pcg& g, const U128 inc, const ZigLds* z) {
     double zz = zig_normal64(g, inc, z);
-    double u = g.next_f64(inc);
-    bool jump = u < c.jump_prob;
+    bool jump = (g.next(inc) >> 11) < c.jump_t53;  // next_f64() < jump_prob
     double zj = jump ? zig_normal64(g, inc, z) : 0.0;
     apply(zz, jump, zj);
     return s;
@@ -172,8 +175,7 @@ struct Merton32 {
   }
   __device__ __forceinline__ float step(Pcg& g, const U128 inc, const ZigLds* z) {
     float zz = zig_normal32(g, inc, z);
-    double u = g.next_f64(inc);  // exact, same comparison as FP64
-    bool jump = u < c.jump_prob;
+    bool jump = (g.next(inc) >> 11) < c.jump_t53;  // same comparison as FP64
     float zj = jump ? zig_normal32(g, inc, z) : 0.f;
     apply(zz, jump, zj);
     return ls;
