@@ -21,13 +21,16 @@ namespace stocha {
 template <bool kFp32, uint32_t kPer>
 struct NormalSrc {
   using Real = typename ZigFeed<kFp32>::Real;
+  using Tables = typename ZigFeed<kFp32>::Tables;
   using Rec = Real;
   static constexpr uint32_t kPerStep = kPer;
+  static constexpr uint32_t kDefer = 8;  // slow-block deferral (queue_run)
   static constexpr int kWaves = kFp32 ? 5 : 4;  // occupancy target of the fused kernel
   ZigFeed<kFp32> zf;
   template <class C>
   __device__ __forceinline__ explicit NormalSrc(const C&) {}
-  __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* z, Rec& r, bool active) {
+  __device__ __forceinline__ bool pending() const { return zf.pending(); }
+  __device__ __forceinline__ bool feed(uint64_t u, const Tables* z, Rec& r, bool active) {
     return zf.feed(u, z, r, active);
   }
   template <uint32_t kSlots, int kCols, class M>
@@ -50,10 +53,14 @@ using HestonSrc = NormalSrc<kFp32, 2>;
 template <bool kFp32, class M>
 struct MertonSrc {
   using Real = typename ZigFeed<kFp32>::Real;
+  using Tables = typename ZigFeed<kFp32>::Tables;
   struct Rec {
     Real z, jr;
   };
   static constexpr uint32_t kPerStep = 1;
+  // no slow-block deferral: with a step's draws split between the ziggurat
+  // and the uniform, waiting lanes cost more draws than the shared pass saves
+  static constexpr uint32_t kDefer = 0;
   static constexpr int kWaves = 4;
   ZigFeed<kFp32> zf;
   uint32_t phase = 0;  // 0: z next, 1: uniform next, 2: jump normal next
@@ -62,9 +69,11 @@ struct MertonSrc {
   M jm;          // for jump_return()
   template <class C>
   __device__ __forceinline__ explicit MertonSrc(const C& c) : t53(c.jump_t53), jm(c, false) {}
+  // (phase 1 is entered with the ziggurat idle, so only its states count)
+  __device__ __forceinline__ bool pending() const { return zf.pending(); }
   // Straight-line: the phase transitions and the entry are selects; only the
   // ziggurat's own slow block branches.  rec is written on every call.
-  __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* z, Rec& r, bool active) {
+  __device__ __forceinline__ bool feed(uint64_t u, const Tables* z, Rec& r, bool active) {
     Real x;
     const bool uni = active & (phase == 1u);
     const bool got = zf.feed(u, z, x, active & !uni);
@@ -106,7 +115,7 @@ struct NoHook {
 // caller reads afterwards).
 template <class Src, uint32_t kSlots, int kCols, class Model, class Hook = NoHook>
 __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128 inc,
-                                          uint32_t steps, const ZigLds* zig,
+                                          uint32_t steps, const typename Src::Tables* zig,
                                           typename Src::Rec (*q)[kCols], Hook hook = Hook()) {
   static_assert((kSlots & (kSlots - 1)) == 0 && kSlots > Src::kPerStep, "ring size");
   constexpr uint32_t kCap = kSlots - 1;
@@ -128,7 +137,19 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
     // bottom-tested: one draw per step is always taken (by lanes with room),
     // which keeps the loop-carried state in place across the test
     do {
-      const bool active = cnt < kCap;
+      // A lane whose next draw goes to the ziggurat's slow block holds it
+      // until Src::kDefer lanes are waiting or one of the waiting lanes has run
+      // dry, so that one pass through the slow block serves many lanes
+      // (without the wait, 69% of a wave's draws would take it).  The lane's
+      // own draws are consumed in the same order either way.
+      bool active = cnt < kCap;
+      if constexpr (Src::kDefer > 0) {
+        const bool pend = src.pending();
+        const uint64_t waiting = __builtin_amdgcn_ballot_w64(pend);
+        const uint64_t dry = waiting & __builtin_amdgcn_ballot_w64(cnt < kNeed);
+        const bool go = ((uint32_t)__builtin_popcountll(waiting) >= Src::kDefer) | (dry != 0);
+        active = active & (!pend | go);
+      }
       const uint64_t u = g.output();
       const U128 sn = g.stepped(incv);
       g.s.lo = active ? sn.lo : g.s.lo;
@@ -147,7 +168,8 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
 
 template <class Src, class Model, uint32_t kSlots, int kCols, class Const, class Hook = NoHook>
 __device__ __forceinline__ double queue_path(const Const& c, Pcg& g, const U128 inc,
-                                             uint32_t steps, const ZigLds* zig,
+                                             uint32_t steps,
+                                             const typename Src::Tables* zig,
                                              typename Src::Rec (*q)[kCols], bool mirror = false,
                                              Hook hook = Hook()) {
   Src src(c);

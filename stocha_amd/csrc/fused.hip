@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
     const Payoff pay, double* __restrict__ partials, unsigned* __restrict__ counter,
     double* __restrict__ out) {
   using Rec = typename Src::Rec;
-  __shared__ ZigLds zig;
+  __shared__ typename Src::Tables zig;
   __shared__ Rec q[kQueue][kFusedBlock];
   __shared__ double red[kFusedBlock / 64][kMaxAcc];
   zig_load(&zig);
@@ -271,17 +271,17 @@ int launch_fused(int model, int precision, int schedule, const void* mconst,
   switch (model) {
     case kGbm: {
       const GbmConst& c = *(const GbmConst*)mconst;
-      return f32 ? run_terminal<Gbm32, GbmSrc<true>, 8, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+      return f32 ? run_terminal<Gbm32, GbmSrc<true>, 16, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
                  : run_terminal<Gbm64, GbmSrc<false>, 8, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     case kHeston: {
       const HestonConst& c = *(const HestonConst*)mconst;
-      return f32 ? run_terminal<Heston32, HestonSrc<true>, 8, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+      return f32 ? run_terminal<Heston32, HestonSrc<true>, 16, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
                  : run_terminal<Heston64, HestonSrc<false>, 8, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     case kMerton: {
       const MertonConst& c = *(const MertonConst*)mconst;
-      return f32 ? run_terminal<Merton32, MertonSrc<true, Merton32>, 4, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
+      return f32 ? run_terminal<Merton32, MertonSrc<true, Merton32>, 8, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
                  : run_terminal<Merton64, MertonSrc<false, Merton64>, 4, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
     }
     default:

@@ -166,13 +166,17 @@ struct ZigFast32 {
   float xs;     // +-X[l] * 2^-32, rounded to FP32
 };
 
-struct ZigLds {
-  ZigFast fast[512];
+// What the FP32 samplers read (12 KB) ...
+struct ZigLds32 {
   ZigFast32 fast32[512];
   uint64_t t[256];    // T[l], exact fast-path threshold on m = u >> 11
   float2 ydy32[256];  // FP32 wedge: (Y[l], Y[l+1] - Y[l])
   double x[257];
   double y[257];
+};
+// ... and the FP64 fast-path table on top (20 KB in all).
+struct ZigLds : ZigLds32 {
+  ZigFast fast[512];
 };
 
 __constant__ const double kZigX[257] = {ZIG_X_VALUES};
@@ -181,14 +185,11 @@ __constant__ const uint64_t kZigT[256] = {ZIG_T_VALUES};
 constexpr double kZigR = ZIG_R_VALUE;
 
 // Cooperative fill; caller must __syncthreads() afterwards.
-__device__ __forceinline__ void zig_load(ZigLds* z) {
+__device__ __forceinline__ void zig_load(ZigLds32* z) {
   for (int i = threadIdx.x; i < 512; i += blockDim.x) {
     const int l = i & 255;
     const double xv = (i & 256) ? -kZigX[l] : kZigX[l];
     const uint32_t th = (uint32_t)(kZigT[l] >> 21);
-    z->fast[i].th = th;
-    z->fast[i].pad = 0u;
-    z->fast[i].x = xv;
     z->fast32[i].th = th;
     z->fast32[i].xs = (float)(xv * 0x1p-32);
     if (i < 256) {
@@ -199,6 +200,15 @@ __device__ __forceinline__ void zig_load(ZigLds* z) {
       z->x[i] = kZigX[i];
       z->y[i] = kZigY[i];
     }
+  }
+}
+__device__ __forceinline__ void zig_load(ZigLds* z) {
+  zig_load(static_cast<ZigLds32*>(z));
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    const int l = i & 255;
+    z->fast[i].th = (uint32_t)(kZigT[l] >> 21);
+    z->fast[i].pad = 0u;
+    z->fast[i].x = (i & 256) ? -kZigX[l] : kZigX[l];
   }
 }
 
@@ -305,7 +315,7 @@ __device__ __forceinline__ double zig_normal64(Pcg& g, const U128 inc, const Zig
 // hardware exp2 in FP32; when |y - e| is within 1e-5*e of the boundary it
 // recomputes exp in FP64, so the accept/reject decision equals the FP64
 // form's (the FP32 exp error is below 1e-6 relative for |x| <= R).
-__device__ __forceinline__ float zig_normal32(Pcg& g, const U128 inc, const ZigLds* __restrict__ z) {
+__device__ __forceinline__ float zig_normal32(Pcg& g, const U128 inc, const ZigLds32* __restrict__ z) {
   for (;;) {
     const uint64_t u = g.next(inc);
     const uint32_t idx = (uint32_t)u & 0x1ffu, layer = idx & 0xffu;

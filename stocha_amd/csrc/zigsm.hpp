@@ -8,8 +8,8 @@
 // pays ~2 PCG draws per normal.  Here every call consumes exactly the one u64
 // it is given, and each lane decides what that draw is from its own state:
 //   0 attempt  -> fast accept (normal done), wedge pending, or tail pending
-//   1 wedge    -> the draw is the wedge uniform: accept, or back to attempt
-//   2, 3 tail  -> the draws are the tail sampler's (u1, u2) pair
+//   2 wedge    -> the draw is the wedge uniform: accept, or back to attempt
+//   1, 3 tail  -> the draws are the tail sampler's (u1, u2) pair
 // A lane's draws are consumed in exactly the order of normal.rs:18-67, so
 // values are unchanged; only the wave's schedule differs.  Since a call never
 // draws by itself, the caller's generator step is unconditional and can
@@ -32,10 +32,15 @@ namespace stocha {
 template <bool kFp32>
 struct ZigFeed {
   using Real = typename std::conditional<kFp32, float, double>::type;
-  uint32_t state = 0;  // 0 attempt, 1 wedge uniform next, 2 / 3 tail pair
+  using Tables = typename std::conditional<kFp32, ZigLds32, ZigLds>::type;
+  uint32_t state = 0;  // 0 attempt, 2 wedge uniform next, 1 / 3 tail pair
   uint32_t idx = 0;    // table index (layer | sign << 8) of a pending wedge / tail
   uint64_t ua = 0;     // wedge: the attempt draw; tail: the pair's first draw
   Real xw = 0;         // wedge: the attempt's (signed) x
+
+  // The next draw goes to the slow block (a wedge uniform or the second draw
+  // of a tail pair): states 2 and 3.
+  __device__ __forceinline__ bool pending() const { return state >= 2u; }
 
   // Consumes draw u when `active` (state unchanged otherwise).  Straight-line
   // on every lane: the table read, the fast test and value, and the cheap
@@ -44,7 +49,7 @@ struct ZigFeed {
   // evaluated -- a wedge uniform or the second draw of a tail pair -- take
   // the divergent block (feed_eval).  `out` is written on every call and
   // holds a normal only when the call returns true.
-  __device__ __forceinline__ bool feed(uint64_t u, const ZigLds* __restrict__ z, Real& out,
+  __device__ __forceinline__ bool feed(uint64_t u, const Tables* __restrict__ z, Real& out,
                                        bool active = true) {
     const uint32_t i = (uint32_t)u & 0x1ffu;
     const uint32_t hi = (uint32_t)(u >> 32);
@@ -64,18 +69,18 @@ struct ZigFeed {
     // a tie on the high word (one draw in 2^32) is booked as a miss here and
     // settled with the full threshold in feed_eval
     const bool tie = at & (hi == th);
-    const bool slow = (active & ((state == 1u) | (state == 3u))) | tie;
+    const bool slow = (active & pending()) | tie;
     const bool miss = at & !take;
-    const bool tail1 = active & (state == 2);
+    const bool tail1 = active & (state == 1u);
     ua = (miss | tail1) ? u : ua;
     idx = miss ? i : idx;
     xw = miss ? out : xw;
-    state = miss ? ((i & 0xffu) == 0 ? 2u : 1u) : (tail1 ? 3u : state);
+    state = miss ? ((i & 0xffu) == 0 ? 1u : 2u) : (tail1 ? 3u : state);
     if (slow) return feed_eval(u, z, out, tie);
     return take;
   }
 
-  __device__ __forceinline__ bool feed_eval(uint64_t u, const ZigLds* __restrict__ z, Real& out,
+  __device__ __forceinline__ bool feed_eval(uint64_t u, const Tables* __restrict__ z, Real& out,
                                             bool tie) {
     if (tie) {  // exact fast-path test (normal.rs:26-29); out holds x
       if ((u >> 11) < z->t[idx & 0xffu]) {
@@ -84,7 +89,7 @@ struct ZigFeed {
       }
       return false;  // a real miss, already booked
     }
-    if (state == 1) {  // wedge (normal.rs:36-42): u is the uniform
+    if (state == 2u) {  // wedge (normal.rs:36-42): u is the uniform
       state = 0;
       const uint32_t l = idx & 0xffu;
       out = xw;
@@ -103,7 +108,8 @@ struct ZigFeed {
         return y < exp(-0.5 * xw * xw);
       }
     }
-    state = 2;  // tail (normal.rs:54-67): the pair's second draw
+    state = 1;  // tail (normal.rs:54-67): u is the pair's second draw; on a
+                // rejection the next draw starts a new pair
     Real t;
     bool ok;
     if constexpr (kFp32) {
