@@ -63,11 +63,11 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchInstDev* __res
                                                        const StreamDesc* __restrict__ descs,
                                                        uint32_t* __restrict__ counter,
                                                        double* __restrict__ tile_sums) {
-  __shared__ ZigLds zig;
+  __shared__ ZigLds32 zig;
   __shared__ uint32_t cur;
   __shared__ double red[kBlock / 64][2];
-  // step-input rings: 32 bytes per lane (8 GBM/Heston normals, 4 Merton entries)
-  __shared__ float2 qbuf[4][kBlock];
+  // step-input rings: 64 bytes per lane (16 GBM/Heston normals, 8 Merton entries)
+  __shared__ float2 qbuf[8][kBlock];
   zig_load(&zig);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (;;) {
@@ -89,13 +89,13 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchInstDev* __res
       Pcg g{stream_at(&sd, tl.path0 + (valid ? k : 0))};
       double st;
       if (in.model == kGbm)
-        st = queue_path<GbmSrc<true>, Gbm32, 8, kBlock>(in.g, g, inc, in.steps, &zig,
+        st = queue_path<GbmSrc<true>, Gbm32, 16, kBlock>(in.g, g, inc, in.steps, &zig,
                                                         (GbmSrc<true>::Rec(*)[kBlock])qbuf);
       else if (in.model == kHeston)
-        st = queue_path<HestonSrc<true>, Heston32, 8, kBlock>(
+        st = queue_path<HestonSrc<true>, Heston32, 16, kBlock>(
             in.h, g, inc, in.steps, &zig, (HestonSrc<true>::Rec(*)[kBlock])qbuf);
       else
-        st = queue_path<MertonSrc<true, Merton32>, Merton32, 4, kBlock>(
+        st = queue_path<MertonSrc<true, Merton32>, Merton32, 8, kBlock>(
             in.m, g, inc, in.steps, &zig, (MertonSrc<true, Merton32>::Rec(*)[kBlock])qbuf);
       if (valid) {
         double pay = st - in.strike;

@@ -80,8 +80,9 @@ __global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, c
                                                             const int is_put) {
   using Src = GbmSrc<kFp32>;
   using Model = typename std::conditional<kFp32, Gbm32, Gbm64>::type;
-  __shared__ ZigLds zig;
-  __shared__ typename Src::Rec q[8][kLsBlock];
+  constexpr uint32_t kSlots = kFp32 ? 16 : 8;
+  __shared__ typename Src::Tables zig;
+  __shared__ typename Src::Rec q[kSlots][kLsBlock];
   zig_load(&zig);
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * kLsBlock;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, c
     hook.valid = i < n;
     hook.s0f = (float)mc.s0;
     Pcg g{stream_at(&sd, hook.valid ? i : 0)};
-    (void)queue_path<Src, Model, 8, kLsBlock>(mc, g, sd.inc, steps, &zig, q, false, hook);
+    (void)queue_path<Src, Model, kSlots, kLsBlock>(mc, g, sd.inc, steps, &zig, q, false, hook);
     if (hook.valid) {
       double last;
       if constexpr (kFp32)

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kPathBlock) void paths_kernel(const Const mc, const
                                                            const uint32_t steps,
                                                            double* __restrict__ out) {
   using Rec = typename Src::Rec;
-  __shared__ ZigLds zig;
+  __shared__ typename Src::Tables zig;
   __shared__ Rec q[kSlots][kPathBlock];
   __shared__ double tile[kTile][kPathBlock + 1];
   zig_load(&zig);
