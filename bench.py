@@ -203,6 +203,26 @@ def main():
                                  "path_steps_per_s_kernel": gsteps / (kg * 1e-3),
                                  "hbm_gbs_kernel": gkw["n_paths"] * 253 * 8 / (kg * 1e-3) / 1e9}
 
+        # bump-and-revalue Greeks, GBM call 1,048,576 x 252, all five Greeks:
+        # 8 bump scenarios in FP64 on shared draws (one launch), against the
+        # same 8 scenarios priced one launch each (FP64 fused European kernel)
+        fkw = dict(model="gbm", params={"s0": 100.0, "r": R, "sigma": 0.2, "t": T},
+                   payoff="call", strike=100.0, n_paths=N_PATHS, n_steps=STEPS,
+                   greeks=["delta", "gamma", "vega", "theta", "rho"], seed=SEED)
+        tf = time_calls(lambda: S.greeks_fd(**fkw), 3, 1)
+        kf = kernel_ms(S, lambda: S.greeks_fd(**fkw))
+
+        def one_per_launch():
+            for _ in range(8):
+                S.european_price("gbm", {"s0": 100.0, "mu": R, "sigma": 0.2}, T, STEPS, N_PATHS,
+                                 [100.0], r=R, seed=SEED, precision=64)
+        t1 = time_calls(one_per_launch, 2, 1)
+        out["greeks_fd_gbm_1Mx252"] = {
+            "ms": tf * 1e3, "kernel_ms": kf, "scenarios": 8,
+            "scenario_path_steps_per_s": 8 * N_PATHS * STEPS / tf,
+            "ms_one_launch_per_scenario": t1 * 1e3, "speedup": t1 / tf,
+            "values": S.greeks_fd(**fkw)}
+
         # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
         from stocha_amd import batch as B
         inst = B.config4_instances(4096, seed=SEED)

@@ -12,6 +12,8 @@
  *   stocha_multi_gbm             <- src/lib.rs:759-827   (multi_gbm)
  *   stocha_lsmc_american_option  <- src/lib.rs:854-918   (lsmc_american_option)
  *   stocha_greeks_pathwise       <- src/lib.rs:1177-1242 (greeks_pathwise)
+ *   stocha_greeks_fd             <- src/lib.rs:1064-1156 (greeks_fd, built-in payoffs)
+ *   stocha_terminal_prices       <- src/finance/greeks.rs:91-167 (greeks_fd, callable payoffs)
  *
  * Conventions (all entry points):
  *   - Plain C types only.  Integer arguments the reference takes as usize/u64
@@ -132,6 +134,33 @@ int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double s
                            int is_call, uint64_t n_paths, uint64_t n_steps,
                            const char *const *greeks, uint64_t n_greeks, uint64_t seed,
                            double *values, char *err, size_t errlen);
+
+/* src/lib.rs:1064-1117 -> greeks.rs:227-358 (greeks_fd with the built-in
+ * payoffs "call" / "put"; other text raises the reference's "Unknown payoff"
+ * message).  model: STOCHA_MODEL_*; params, in the order of the reference's
+ * params dict (lib.rs:958-1038):
+ *   GBM    {s0, r, sigma, t}                          (n_params 4)
+ *   Heston {s0, v0, r, kappa, theta, xi, rho, t}      (n_params 8)
+ *   Merton {s0, r, sigma, lambda_, mu_j, sigma_j, t}  (n_params 7)
+ * scheme (Heston only; NULL = "euler"; "qe" -> STOCHA_ENOTIMPL).  greeks:
+ * n_greeks names among delta/gamma/vega/theta/rho (checked first, lib.rs:920);
+ * values[i] receives the i-th.  Bumps max(|p| * bump_size, 1e-8); every
+ * scenario on the same seed (common random numbers); scenarios that draw
+ * alike share one launch and one stream of draws.  FP64 reference
+ * arithmetic throughout. */
+int stocha_greeks_fd(int model, const double *params, uint64_t n_params, const char *scheme,
+                     const char *payoff, double strike, uint64_t n_paths, uint64_t n_steps,
+                     const char *const *greeks, uint64_t n_greeks, uint64_t seed,
+                     double bump_size, double *values, char *err, size_t errlen);
+
+/* greeks.rs:91-167 (ModelSpec::terminal_prices): S_T of paths 0..n_paths-1
+ * of `model` (params and scheme as stocha_greeks_fd; GBM with q = 0 and no
+ * antithetic paths), FP64, into out (n_paths, host or device per `where`;
+ * out == NULL validates only).  The callable-payoff form of greeks_fd
+ * (lib.rs:1118-1150) evaluates the payoff on these. */
+int stocha_terminal_prices(int model, const double *params, uint64_t n_params, const char *scheme,
+                           uint64_t n_steps, uint64_t n_paths, uint64_t seed, double *out,
+                           int where, char *err, size_t errlen);
 
 /* ---- forms beside the drop-in (no reference counterpart; they extend the
  *      same path and use the same streams) ------------------------------- */

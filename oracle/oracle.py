@@ -58,6 +58,8 @@ def lib():
                                      _pd]
         L.orc_lstsq.argtypes = [_u, _u, _pd, _pd, _pd]
         L.orc_greeks_pathwise.argtypes = [_d, _d, _d, _d, _d, _i, _u, _u, _u, _pd, _pd]
+        L.orc_greeks_fd.argtypes = [_i, _pd, _d, _d, _u, _u, C.POINTER(C.c_int), _i, _u, _d,
+                                    _pd]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -216,6 +218,21 @@ def greeks_pathwise(s0, r, sigma, t, strike, is_call, n_paths, n_steps, seed=42)
     lib().orc_greeks_pathwise(s0, r, sigma, t, strike, int(is_call), n_paths, n_steps, seed,
                               _p(d), _p(v))
     return float(d[0]), float(v[0])
+
+
+_GREEK_IDS = {"delta": 0, "gamma": 1, "vega": 2, "theta": 3, "rho": 4}
+
+
+def greeks_fd(model, spec, payoff, strike, n_paths, n_steps, greeks, seed=42, bump_size=0.01):
+    """greeks_fd_core (greeks.rs:227-358) with a built-in payoff; model id
+    0/1/2, spec in the params-dict order.  Returns {name: value}."""
+    sp = np.ascontiguousarray(spec, np.float64)
+    ids = (C.c_int * max(1, len(greeks)))(*[_GREEK_IDS[g] for g in greeks])
+    out = np.zeros(max(1, len(greeks)))
+    rc = lib().orc_greeks_fd(model, _p(sp), 1.0 if payoff == "call" else -1.0, strike, n_paths,
+                             n_steps, ids, len(greeks), seed, bump_size, _p(out))
+    assert rc == 0
+    return {g: float(out[i]) for i, g in enumerate(greeks)}
 
 
 def num_threads():

@@ -674,3 +674,105 @@ int orc_greeks_pathwise(double s0, double r, double sigma, double t, double stri
     free(term);
     return 0;
 }
+
+/* ---- bump-and-revalue Greeks (greeks.rs:227-358) ----
+ * spec: the ModelSpec fields in the reference's params-dict order (GBM: s0, r,
+ * sigma, t; Heston: s0, v0, r, kappa, theta, xi, rho, t; Merton: s0, r,
+ * sigma, lambda, mu_j, sigma_j, t).  sign +1 call / -1 put (greeks.rs:12-17).
+ * greeks: ids 0 delta, 1 gamma, 2 vega, 3 theta, 4 rho. */
+enum { ORC_S0 = 0, ORC_SIGMA = 1, ORC_T = 2, ORC_MU = 3 };
+
+/* ModelSpec::param_value / with_bumped (greeks.rs:169-224): field index of
+ * S0, Sigma (Heston: v0), T, Mu (the rate) */
+static int orc_fd_index(int model, int param) {
+    static const int idx[3][4] = {{0, 2, 3, 1}, {0, 1, 7, 2}, {0, 2, 6, 1}};
+    return idx[model][param];
+}
+
+/* mc_price (greeks.rs:304-311): terminal_prices (greeks.rs:91-167, GBM with
+ * q = 0), Payoff::eval, exp(-r t) * (left-to-right sum) / n. */
+static double orc_mc_price(int model, const double *spec, double sign, double strike,
+                           uint64_t steps, uint64_t n_paths, uint64_t seed, double *term) {
+    double p[8];
+    double t;
+    if (model == 0) {
+        p[0] = spec[0]; p[1] = spec[1]; p[2] = spec[2]; p[3] = 0.0;
+        t = spec[3];
+    } else if (model == 1) {
+        for (int i = 0; i < 7; i++) p[i] = spec[i];
+        t = spec[7];
+    } else {
+        for (int i = 0; i < 6; i++) p[i] = spec[i];
+        t = spec[6];
+    }
+    orc_terminals(model, p, t, steps, n_paths, seed, term);
+    double sum = 0.0;
+    for (uint64_t i = 0; i < n_paths; i++) {
+        double v = sign > 0 ? term[i] - strike : strike - term[i];
+        sum += v > 0.0 ? v : 0.0;   /* f64::max(0.0); a NaN payoff counts 0 */
+    }
+    double r = spec[orc_fd_index(model, ORC_MU)];
+    double discount = exp(-r * t);
+    return discount * sum / (double)n_paths;
+}
+
+int orc_greeks_fd(int model, const double *spec, double sign, double strike, uint64_t n_paths,
+                  uint64_t steps, const int *greeks, int n_greeks, uint64_t seed,
+                  double bump_size, double *values) {
+    if (model < 0 || model > 2) return 1;
+    const int nspec = model == 0 ? 4 : model == 1 ? 8 : 7;
+    /* required_scenarios (greeks.rs:231-265): key = param * 2 + (dir < 0), base = 8 */
+    int keys[9], nk = 0;
+#define ORC_ADD(k) do { int _k = (k), _f = 0; for (int _i = 0; _i < nk; _i++) _f |= keys[_i] == _k; \
+                        if (!_f) keys[nk++] = _k; } while (0)
+    for (int g = 0; g < n_greeks; g++) {
+        switch (greeks[g]) {
+            case 0: ORC_ADD(ORC_S0 * 2); ORC_ADD(ORC_S0 * 2 + 1); break;
+            case 1: ORC_ADD(ORC_S0 * 2); ORC_ADD(8); ORC_ADD(ORC_S0 * 2 + 1); break;
+            case 2: ORC_ADD(ORC_SIGMA * 2); ORC_ADD(ORC_SIGMA * 2 + 1); break;
+            case 3: ORC_ADD(ORC_T * 2 + 1); ORC_ADD(8); break;
+            case 4: ORC_ADD(ORC_MU * 2); ORC_ADD(ORC_MU * 2 + 1); break;
+            default: return 1;
+        }
+    }
+#undef ORC_ADD
+    /* compute_bump (greeks.rs:227-229), first occurrence of each parameter */
+    double bump[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nk; i++) {
+        if (keys[i] == 8) continue;
+        int prm = keys[i] / 2;
+        double b = fabs(spec[orc_fd_index(model, prm)]) * bump_size;
+        bump[prm] = b > 1e-8 ? b : 1e-8;
+    }
+    double price[9];
+    double *term = (double *)malloc(sizeof(double) * (n_paths ? n_paths : 1));
+    if (!term) return 1;
+    for (int i = 0; i < nk; i++) {
+        double v[8];
+        for (int j = 0; j < nspec; j++) v[j] = spec[j];
+        if (keys[i] != 8) {
+            int prm = keys[i] / 2;
+            double delta = (keys[i] & 1) ? -bump[prm] : bump[prm];
+            v[orc_fd_index(model, prm)] += delta;  /* with_bumped */
+        }
+        price[keys[i]] = orc_mc_price(model, v, sign, strike, steps, n_paths, seed, term);
+    }
+    free(term);
+    /* compute_greeks_from_prices (greeks.rs:267-302) */
+    for (int g = 0; g < n_greeks; g++) {
+        double h;
+        switch (greeks[g]) {
+            case 0: h = bump[ORC_S0];
+                values[g] = (price[0] - price[1]) / (2.0 * h); break;
+            case 1: h = bump[ORC_S0];
+                values[g] = (price[0] - 2.0 * price[8] + price[1]) / (h * h); break;
+            case 2: h = bump[ORC_SIGMA];
+                values[g] = (price[2] - price[3]) / (2.0 * h); break;
+            case 3: h = bump[ORC_T];
+                values[g] = (price[5] - price[8]) / h; break;
+            default: h = bump[ORC_MU];
+                values[g] = (price[6] - price[7]) / (2.0 * h); break;
+        }
+    }
+    return 0;
+}

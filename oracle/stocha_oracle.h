@@ -80,6 +80,14 @@ int orc_greeks_pathwise(double s0, double r, double sigma, double t, double stri
                         uint64_t n_paths, uint64_t steps, uint64_t seed, double *delta,
                         double *vega);
 
+/* ---- bump-and-revalue Greeks (greeks.rs:227-358) ----
+ * spec in the reference's params-dict order per model (GBM 4, Heston 8,
+ * Merton 7 values); sign +1 call, -1 put; greeks ids 0..4 = delta, gamma,
+ * vega, theta, rho.  Prices with left-to-right payoff sums (mc_price). */
+int orc_greeks_fd(int model, const double *spec, double sign, double strike, uint64_t n_paths,
+                  uint64_t steps, const int *greeks, int n_greeks, uint64_t seed,
+                  double bump_size, double *values);
+
 /* Number of OpenMP threads the oracle will use. */
 int orc_num_threads(void);
 

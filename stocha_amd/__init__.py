@@ -10,6 +10,7 @@ errors:
     multi_gbm              (__init__.py:574-627)
     lsmc_american_option   (__init__.py:630-671)
     greeks_pathwise        (__init__.py:725-769)
+    greeks_fd              (__init__.py:674-722; Heston scheme "euler")
 
 They run on the GPU through the C ABI of libstocha_gpu.so (hand-written HIP
 kernels); there is no CPU fallback.  Forms beside the drop-in API:
@@ -20,7 +21,7 @@ kernels); there is no CPU fallback.  Forms beside the drop-in API:
                            (stocha_amd.autograd wraps it as a torch.autograd.Function)
 
 The reference's other functions (RNG, sobol, halton, copulas, Hull-White,
-SABR, Heston COS / calibration, SSVI, exotics, var_cvar, greeks_fd) are not
+SABR, Heston COS / calibration, SSVI, exotics, var_cvar) are not
 part of this package: calling them raises NotImplementedError.
 """
 from __future__ import annotations
@@ -42,6 +43,7 @@ __all__ = [
     "multi_gbm",
     "lsmc_american_option",
     "greeks_pathwise",
+    "greeks_fd",
     "european_price",
     "lsmc_american_option_ex",
     "basket_price",
@@ -106,6 +108,23 @@ def greeks_pathwise(s0: float, r: float, sigma: float, t: float, strike: float, 
     """Pathwise (IPA) delta and vega for GBM (reference: python/stocha/__init__.py:725-769)."""
     return _S.greeks_pathwise(s0=s0, r=r, sigma=sigma, t=t, strike=strike, is_call=is_call,
                               n_paths=n_paths, n_steps=n_steps, greeks=greeks, seed=seed)
+
+
+def greeks_fd(model: str, params: dict, payoff, strike: float, n_paths: int, n_steps: int,
+              greeks: list[str], seed: int = 42, bump_size: float = 0.01) -> dict[str, float]:
+    """Bump-and-revalue Greeks with common random numbers (reference:
+    python/stocha/__init__.py:674-722).
+
+    model "gbm" / "heston" / "merton"; params as the reference's (GBM: s0, r,
+    sigma, t; Heston: s0, v0, r, kappa, theta, xi, rho, t [, scheme]; Merton:
+    s0, r, sigma, lambda_, mu_j, sigma_j, t); payoff "call", "put" or a callable
+    f(terminals) -> values.  Built-in payoffs price every bump scenario in
+    one launch on shared draws; a callable gets each scenario's FP64 terminal
+    prices from the GPU.
+    """
+    return _S.greeks_fd(model=model, params=params, payoff=payoff, strike=strike,
+                        n_paths=n_paths, n_steps=n_steps, greeks=greeks, seed=seed,
+                        bump_size=bump_size)
 
 
 # ------------------------------------------------------------------ extras --
@@ -191,7 +210,7 @@ def _not_here(name):
 
 
 for _name in ("RNG", "sobol", "halton", "var_cvar", "gaussian_copula", "student_t_copula",
-              "hull_white", "sabr_implied_vol", "sabr_calibrate", "greeks_fd", "heston_price",
+              "hull_white", "sabr_implied_vol", "sabr_calibrate", "heston_price",
               "heston_calibrate", "ssvi_calibrate", "ssvi_implied_vol", "ssvi_local_vol",
               "barrier_price", "asian_price", "lookback_price"):
     globals()[_name] = _not_here(_name)

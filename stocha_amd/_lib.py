@@ -52,6 +52,9 @@ _SIGS = {
                                     C.c_int),
     "stocha_greeks_pathwise": ([_d] * 5 + [_i, _u, _u, C.POINTER(C.c_char_p), _u, _u, _pd, _ps,
                                            _sz], C.c_int),
+    "stocha_greeks_fd": ([_i, _pd, _u, _ps, _ps, _d, _u, _u, C.POINTER(C.c_char_p), _u, _u, _d,
+                          _pd, _ps, _sz], C.c_int),
+    "stocha_terminal_prices": ([_i, _pd, _u, _ps, _u, _u, _u, _vp, _i, _ps, _sz], C.c_int),
     "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _pd, _pd, _pd,
                                _ps, _sz], C.c_int),
     "stocha_lsmc_ex": ([_d] * 5 + [_u, _u, _i, _u, _u, _i, _pd, _pd, _ps, _sz], C.c_int),

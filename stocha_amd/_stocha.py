@@ -184,3 +184,161 @@ def greeks_pathwise(s0, r, sigma, t, strike, is_call, n_paths, n_steps, greeks, 
     for i, n in enumerate(names):
         res[n] = float(out[i])
     return res
+
+
+# ------------------------------------------------------------- greeks_fd --
+# lib.rs:958-1038 (build_model_spec): the params dict keys per model, in the
+# order the C ABI takes them.
+_FD_MODELS = {"gbm": (L.MODEL_GBM, ("s0", "r", "sigma", "t")),
+              "heston": (L.MODEL_HESTON, ("s0", "v0", "r", "kappa", "theta", "xi", "rho", "t")),
+              "merton": (L.MODEL_MERTON, ("s0", "r", "sigma", "lambda_", "mu_j", "sigma_j",
+                                          "t"))}
+_GREEKS = ("delta", "gamma", "vega", "theta", "rho")
+# ModelSpec::param_value / with_bumped (greeks.rs:169-224): index of the
+# bumped field in the spec (S0, Sigma -- Heston's v0 --, T, Mu -- the rate r)
+_FD_INDEX = {"gbm": (0, 2, 3, 1), "heston": (0, 1, 7, 2), "merton": (0, 2, 6, 1)}
+_S0, _SIGMA, _T, _MU = 0, 1, 2, 3
+
+
+def _parse_greeks(greeks):
+    """lib.rs:920-938."""
+    if not isinstance(greeks, list):
+        raise TypeError(f"argument 'greeks': '{type(greeks).__name__}' object cannot be "
+                        "converted to 'PyList'")
+    out = []
+    for g in greeks:
+        if not isinstance(g, str):
+            raise TypeError(f"'{type(g).__name__}' object cannot be converted to 'PyString'")
+        if g not in _GREEKS:
+            raise ValueError(f"Unknown greek: '{g}'. Supported: delta, gamma, vega, theta, rho")
+        out.append(g)
+    return out
+
+
+def _model_spec(model, params):
+    """lib.rs:958-1038: (model id, field values, scheme)."""
+    if model not in _FD_MODELS:
+        raise ValueError(f"Unknown model: '{model}'. Supported: gbm, heston, merton")
+    mid, keys = _FD_MODELS[model]
+    vals = []
+    for k in keys:
+        if k not in params:
+            raise ValueError(f"Missing required param: '{k}'")
+        vals.append(_f64(params[k], k))
+    scheme = None
+    if model == "heston":
+        scheme = params.get("scheme", "euler")
+        if not isinstance(scheme, str):
+            raise TypeError(f"'{type(scheme).__name__}' object cannot be converted to 'PyString'")
+        if scheme not in ("euler", "qe"):
+            raise ValueError("scheme must be 'euler' or 'qe'")
+    return mid, np.array(vals, dtype=np.float64), scheme
+
+
+def _required_scenarios(greeks):
+    """greeks.rs:231-265: (param, dir) keys, None = Base, first-seen order."""
+    keys = []
+
+    def add(k):
+        if k not in keys:
+            keys.append(k)
+    for g in greeks:
+        if g == "delta":
+            add((_S0, 1)); add((_S0, -1))
+        elif g == "gamma":
+            add((_S0, 1)); add(None); add((_S0, -1))
+        elif g == "vega":
+            add((_SIGMA, 1)); add((_SIGMA, -1))
+        elif g == "theta":
+            add((_T, -1)); add(None)
+        else:
+            add((_MU, 1)); add((_MU, -1))
+    return keys
+
+
+def _greeks_from_prices(greeks, prices, bumps):
+    """greeks.rs:267-302."""
+    res = {}
+    for g in greeks:
+        if g == "delta":
+            h = bumps[_S0]
+            v = (prices[(_S0, 1)] - prices[(_S0, -1)]) / (2.0 * h)
+        elif g == "gamma":
+            h = bumps[_S0]
+            v = (prices[(_S0, 1)] - 2.0 * prices[None] + prices[(_S0, -1)]) / (h * h)
+        elif g == "vega":
+            h = bumps[_SIGMA]
+            v = (prices[(_SIGMA, 1)] - prices[(_SIGMA, -1)]) / (2.0 * h)
+        elif g == "theta":
+            h = bumps[_T]
+            v = (prices[(_T, -1)] - prices[None]) / h
+        else:
+            h = bumps[_MU]
+            v = (prices[(_MU, 1)] - prices[(_MU, -1)]) / (2.0 * h)
+        res[g] = float(v)
+    return res
+
+
+def _terminal_prices(mid, vals, scheme, n_steps, n_paths, seed):
+    out = np.empty(n_paths, dtype=np.float64)
+    lib, eb = L.load(), L.ErrBuf()
+    rc = lib.stocha_terminal_prices(mid, _pd(vals), len(vals),
+                                    None if scheme is None else scheme.encode(), n_steps,
+                                    n_paths, seed, out.ctypes.data, L.STOCHA_HOST, eb.buf, eb.n)
+    L.check(rc, eb)
+    return out
+
+
+def greeks_fd(model, params, payoff, strike, n_paths, n_steps, greeks, seed=42, bump_size=0.01):
+    """src/lib.rs:1064-1156 (signature :1063).  Returns {name: value}."""
+    if not isinstance(model, str):
+        raise TypeError(f"argument 'model': '{type(model).__name__}' object cannot be converted "
+                        "to 'PyString'")
+    if not isinstance(params, dict):
+        raise TypeError(f"argument 'params': '{type(params).__name__}' object cannot be "
+                        "converted to 'PyDict'")
+    strike = _f64(strike, "strike")
+    n_paths, n_steps = _uint(n_paths, "n_paths"), _uint(n_steps, "n_steps")
+    if not isinstance(greeks, list):
+        raise TypeError(f"argument 'greeks': '{type(greeks).__name__}' object cannot be "
+                        "converted to 'PyList'")
+    seed, bump_size = _uint(seed, "seed"), _f64(bump_size, "bump_size")
+    names = _parse_greeks(greeks)
+    mid, vals, scheme = _model_spec(model, params)
+    sch = None if scheme is None else scheme.encode()
+    lib, eb = L.load(), L.ErrBuf()
+    if isinstance(payoff, str):
+        arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        out = np.zeros(max(1, len(names)), dtype=np.float64)
+        rc = lib.stocha_greeks_fd(mid, _pd(vals), len(vals), sch, payoff.encode(), strike,
+                                  n_paths, n_steps, arr, len(names), seed, bump_size, _pd(out),
+                                  eb.buf, eb.n)
+        L.check(rc, eb)
+        return {n: float(out[i]) for i, n in enumerate(names)}
+    if not callable(payoff):
+        raise ValueError("payoff must be 'call', 'put', or a callable")
+    # lib.rs:1118-1150: terminals per scenario, the callable's payoffs, the
+    # price exp(-r t) * sum / len with the reference's left-to-right sum
+    rc = lib.stocha_terminal_prices(mid, _pd(vals), len(vals), sch, n_steps, 0, seed, None,
+                                    L.STOCHA_HOST, eb.buf, eb.n)
+    L.check(rc, eb)
+    keys = _required_scenarios(names)
+    idx = _FD_INDEX[model]
+    bumps = {}
+    for k in keys:
+        if k is not None and k[0] not in bumps:
+            b = abs(float(vals[idx[k[0]]])) * bump_size
+            bumps[k[0]] = b if b > 1e-8 else 1e-8  # compute_bump, greeks.rs:227-229
+    prices = {}
+    for k in keys:
+        v = vals.copy()
+        if k is not None:
+            v[idx[k[0]]] += k[1] * bumps[k[0]]
+        r_rate, mat = float(v[idx[_MU]]), float(v[idx[_T]])
+        terminals = _terminal_prices(mid, v, scheme, n_steps, n_paths, seed)
+        pv = np.asarray(payoff(terminals), dtype=np.float64).reshape(-1)
+        total = float(np.cumsum(pv)[-1]) if pv.size else 0.0
+        discount = np.exp(np.float64(-r_rate * mat))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            prices[k] = float(np.float64(discount * total) / np.float64(pv.size))
+    return _greeks_from_prices(names, prices, bumps)

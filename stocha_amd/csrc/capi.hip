@@ -186,6 +186,78 @@ int store_paths(int model, const void* mc, uint64_t seed, uint64_t block, uint64
   return STOCHA_OK;
 }
 
+// ---- greeks_fd host logic (greeks.rs:24-358, lib.rs:958-1038) -------------
+// A model spec holds the reference's fields in the order of the Python
+// params dict: GBM {s0, r, sigma, t}, Heston {s0, v0, r, kappa, theta, xi,
+// rho, t}, Merton {s0, r, sigma, lambda_, mu_j, sigma_j, t} (r is the drift,
+// ModelSpec's mu).
+enum FdParam : int { kBumpS0 = 0, kBumpSigma = 1, kBumpT = 2, kBumpMu = 3 };
+enum FdGreek : int { kDelta = 0, kGamma, kVega, kTheta, kRho };
+
+constexpr uint64_t kFdParams[3] = {4, 8, 7};
+
+// Index of a bump parameter in the spec (ModelSpec::param_value /
+// with_bumped, greeks.rs:169-224: Heston's "sigma" is v0).
+int fd_index(int model, int param) {
+  static const int idx[3][4] = {{0, 2, 3, 1}, {0, 1, 7, 2}, {0, 2, 6, 1}};
+  return idx[model][param];
+}
+
+struct FdScenario {
+  int param;  // -1 = Base
+  int dir;    // +1 Up, -1 Down
+};
+
+// required_scenarios (greeks.rs:231-265): first-seen order, no duplicates.
+std::vector<FdScenario> fd_required(const std::vector<int>& greeks) {
+  std::vector<FdScenario> set;
+  auto add = [&set](int param, int dir) {
+    for (const FdScenario& k : set)
+      if (k.param == param && (param < 0 || k.dir == dir)) return;
+    set.push_back(FdScenario{param, dir});
+  };
+  for (int g : greeks) {
+    switch (g) {
+      case kDelta: add(kBumpS0, 1); add(kBumpS0, -1); break;
+      case kGamma: add(kBumpS0, 1); add(-1, 0); add(kBumpS0, -1); break;
+      case kVega: add(kBumpSigma, 1); add(kBumpSigma, -1); break;
+      case kTheta: add(kBumpT, -1); add(-1, 0); break;
+      case kRho: add(kBumpMu, 1); add(kBumpMu, -1); break;
+    }
+  }
+  return set;
+}
+
+// compute_bump (greeks.rs:227-229): max(|v| * bump_size, 1e-8); f64::max
+// returns 1e-8 for a NaN product, as the comparison below does.
+double fd_bump(double v, double bump_size) {
+  const double b = fabs(v) * bump_size;
+  return b > 1e-8 ? b : 1e-8;
+}
+
+// The spec's model constants (ModelSpec::terminal_prices, greeks.rs:91-167;
+// GBM with q = 0 and no antithetic paths).
+void fd_consts(int model, const double* v, uint64_t steps, GbmConst* g, HestonConst* h,
+               MertonConst* m) {
+  if (model == kGbm) {
+    *g = gbm_const(v[0], v[1], v[2], v[3], steps, 0.0);
+  } else if (model == kHeston) {
+    *h = heston_const(v, v[7], steps);
+  } else {
+    *m = merton_const(v, v[6], steps);
+  }
+}
+
+int fd_check_scheme(int model, const char* scheme, const Err& err) {
+  if (model != kHeston) return STOCHA_OK;  // only the Heston spec reads it (lib.rs:988-1000)
+  const char* sch = scheme ? scheme : "euler";
+  if (strcmp(sch, "euler") == 0) return STOCHA_OK;
+  if (strcmp(sch, "qe") == 0)
+    return err.set(STOCHA_ENOTIMPL,
+                   "scheme 'qe' is not part of the GPU hot path (only 'euler' is implemented)");
+  return err.set(STOCHA_EINVAL, "scheme must be 'euler' or 'qe'");
+}
+
 }  // namespace
 }  // namespace stocha
 
@@ -404,6 +476,158 @@ int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double s
   const double n = (double)n_paths;
   for (uint64_t i = 0; i < n_greeks; ++i)
     values[i] = strcmp(greeks[i], "delta") == 0 ? discount * dsum / n : discount * vsum / n;
+  return STOCHA_OK;
+}
+
+// src/lib.rs:1064-1117 -> greeks.rs:227-358 (greeks_fd, built-in payoff)
+int stocha_greeks_fd(int model, const double* params, uint64_t n_params, const char* scheme,
+                     const char* payoff, double strike, uint64_t n_paths, uint64_t n_steps,
+                     const char* const* greeks, uint64_t n_greeks, uint64_t seed,
+                     double bump_size, double* values, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  // parse_greeks (lib.rs:920-938) runs first
+  static const char* known[] = {"delta", "gamma", "vega", "theta", "rho"};
+  std::vector<int> gl;
+  for (uint64_t i = 0; i < n_greeks; ++i) {
+    int id = -1;
+    for (int k = 0; k < 5; ++k)
+      if (strcmp(greeks[i], known[k]) == 0) id = k;
+    if (id < 0)
+      return err.set(STOCHA_EINVAL,
+                     "Unknown greek: '%s'. Supported: delta, gamma, vega, theta, rho", greeks[i]);
+    gl.push_back(id);
+  }
+  if (model < 0 || model > 2) return err.set(STOCHA_EINVAL, "unknown model id %d", model);
+  if (n_params != kFdParams[model])
+    return err.set(STOCHA_EINVAL, "model %d takes %llu parameters, got %llu", model,
+                   (unsigned long long)kFdParams[model], (unsigned long long)n_params);
+  if (int rc = fd_check_scheme(model, scheme, err)) return rc;
+  double sign;
+  if (payoff && strcmp(payoff, "call") == 0) {
+    sign = 1.0;
+  } else if (payoff && strcmp(payoff, "put") == 0) {
+    sign = -1.0;
+  } else {
+    return err.set(STOCHA_EINVAL, "Unknown payoff: '%s'. Supported: 'call', 'put', or a callable",
+                   payoff ? payoff : "");
+  }
+  if (gl.empty()) return STOCHA_OK;
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+
+  // scenarios, bumps (greeks_fd_core, greeks.rs:311-340)
+  const std::vector<FdScenario> sc = fd_required(gl);
+  double bump[4] = {0, 0, 0, 0};
+  bool have[4] = {false, false, false, false};
+  for (const FdScenario& k : sc) {
+    if (k.param >= 0 && !have[k.param]) {
+      bump[k.param] = fd_bump(params[fd_index(model, k.param)], bump_size);
+      have[k.param] = true;
+    }
+  }
+  const int ns = (int)sc.size();
+  std::vector<std::vector<double>> spec(ns, std::vector<double>(params, params + n_params));
+  for (int i = 0; i < ns; ++i)
+    if (sc[i].param >= 0) spec[i][fd_index(model, sc[i].param)] += sc[i].dir * bump[sc[i].param];
+  std::vector<GbmConst> gc(ns);
+  std::vector<HestonConst> hc(ns);
+  std::vector<MertonConst> mc(ns);
+  for (int i = 0; i < ns; ++i) fd_consts(model, spec[i].data(), n_steps, &gc[i], &hc[i], &mc[i]);
+
+  // one launch per group of scenarios with the same draw grammar: all of
+  // them for GBM / Heston; for Merton those with the same lambda*dt threshold
+  StreamDesc sd;
+  make_stream_desc(seed, block_size(model, n_steps), &sd);
+  std::vector<double> sum(ns, 0.0);
+  std::vector<bool> done(ns, false);
+  for (int i = 0; i < ns; ++i) {
+    if (done[i]) continue;
+    std::vector<int> grp;
+    for (int j = i; j < ns; ++j)
+      if (!done[j] && (model != kMerton || mc[j].jump_t53 == mc[i].jump_t53)) grp.push_back(j);
+    std::vector<GbmConst> g2;
+    std::vector<HestonConst> h2;
+    std::vector<MertonConst> m2;
+    for (int j : grp) {
+      g2.push_back(gc[j]);
+      h2.push_back(hc[j]);
+      m2.push_back(mc[j]);
+    }
+    const void* cp = model == kGbm ? (const void*)g2.data()
+                     : model == kHeston ? (const void*)h2.data() : (const void*)m2.data();
+    double h[8];
+    if (int rc = launch_fd(model, cp, (int)grp.size(), strike, sign, sd, n_paths, n_steps, h,
+                           globals().stream, err))
+      return rc;
+    for (size_t k = 0; k < grp.size(); ++k) {
+      sum[grp[k]] = h[k];
+      done[grp[k]] = true;
+    }
+  }
+  // mc_price (greeks.rs:304-311): exp(-r t) * sum / n with the scenario's r, t
+  const int it = fd_index(model, kBumpT), imu = fd_index(model, kBumpMu);
+  std::vector<double> price(ns);
+  for (int i = 0; i < ns; ++i) {
+    const double discount = exp(-spec[i][imu] * spec[i][it]);
+    price[i] = discount * sum[i] / (double)n_paths;
+  }
+  auto px = [&](int param, int dir) -> double {
+    for (int i = 0; i < ns; ++i)
+      if (sc[i].param == param && (param < 0 || sc[i].dir == dir)) return price[i];
+    return NAN;
+  };
+  // compute_greeks_from_prices (greeks.rs:267-302)
+  for (uint64_t i = 0; i < n_greeks; ++i) {
+    double v = NAN;
+    switch (gl[i]) {
+      case kDelta: v = (px(kBumpS0, 1) - px(kBumpS0, -1)) / (2.0 * bump[kBumpS0]); break;
+      case kGamma:
+        v = (px(kBumpS0, 1) - 2.0 * px(-1, 0) + px(kBumpS0, -1)) / (bump[kBumpS0] * bump[kBumpS0]);
+        break;
+      case kVega: v = (px(kBumpSigma, 1) - px(kBumpSigma, -1)) / (2.0 * bump[kBumpSigma]); break;
+      case kTheta: v = (px(kBumpT, -1) - px(-1, 0)) / bump[kBumpT]; break;
+      case kRho: v = (px(kBumpMu, 1) - px(kBumpMu, -1)) / (2.0 * bump[kBumpMu]); break;
+    }
+    values[i] = v;
+  }
+  return STOCHA_OK;
+}
+
+// greeks.rs:91-167 (ModelSpec::terminal_prices), for greeks_fd's callable payoff
+// (lib.rs:1118-1150).
+int stocha_terminal_prices(int model, const double* params, uint64_t n_params, const char* scheme,
+                           uint64_t n_steps, uint64_t n_paths, uint64_t seed, double* out,
+                           int where, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  if (model < 0 || model > 2) return err.set(STOCHA_EINVAL, "unknown model id %d", model);
+  if (n_params != kFdParams[model])
+    return err.set(STOCHA_EINVAL, "model %d takes %llu parameters, got %llu", model,
+                   (unsigned long long)kFdParams[model], (unsigned long long)n_params);
+  if (int rc = fd_check_scheme(model, scheme, err)) return rc;
+  if (!out || n_paths == 0) return STOCHA_OK;
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  GbmConst g;
+  HestonConst h;
+  MertonConst m;
+  fd_consts(model, params, n_steps, &g, &h, &m);
+  const void* cp = model == kGbm ? (const void*)&g : model == kHeston ? (const void*)&h
+                                                                      : (const void*)&m;
+  StreamDesc sd;
+  make_stream_desc(seed, block_size(model, n_steps), &sd);
+  hipStream_t st = globals().stream;
+  if (where == STOCHA_DEVICE) {
+    if (int rc = launch_terminals(model, cp, sd, n_paths, n_steps, out, st, err)) return rc;
+    STOCHA_HIP(hipStreamSynchronize(st));
+    return STOCHA_OK;
+  }
+  Workspace* ws = nullptr;
+  if (int rc = workspace(sizeof(double) * n_paths, 0, &ws, err)) return rc;
+  if (int rc = launch_terminals(model, cp, sd, n_paths, n_steps, static_cast<double*>(ws->dev),
+                                st, err))
+    return rc;
+  STOCHA_HIP(hipMemcpyAsync(out, ws->dev, sizeof(double) * n_paths, hipMemcpyDeviceToHost, st));
+  STOCHA_HIP(hipStreamSynchronize(st));
   return STOCHA_OK;
 }
 

@@ -17,6 +17,9 @@
 //   kScheduleFeed  per-lane draw-feed state machine (zigsm.hpp) filling a
 //                  small LDS queue of step inputs; the wave applies each step
 //                  with all lanes together (terminal_feed_kernel).
+#include <type_traits>
+#include <utility>
+
 #include "host.hpp"
 #include "drivers.hpp"
 
@@ -25,6 +28,7 @@ namespace stocha {
 constexpr int kFusedBlock = 256;
 constexpr int kMaxStrikes = 8;
 constexpr int kMaxAcc = 2 * kMaxStrikes;
+constexpr int kMaxScen = 8;  // bump scenarios per launch (greeks_fd)
 
 struct CallPayoff {
   double k[kMaxStrikes];
@@ -305,6 +309,187 @@ int run_greeks_pathwise(const GbmConst& gc, const StreamDesc& sd, uint64_t n_pat
   *delta_sum = h[0];
   *vega_sum = h[1];
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Bump-and-revalue Greeks (greeks.rs:227-358) on shared draws.  Every
+// scenario of greeks_fd_core prices the same paths with the same seed and
+// block size (common random numbers), and a bump of s0, sigma / v0, mu or t
+// leaves the sequence of draws a path consumes unchanged -- except Merton's t
+// bump, which moves lambda*dt and so which uniforms jump.  One launch
+// therefore steps up to kMaxScen scenarios of a path on one stream of draws:
+// the queue entries are produced once and applied to every scenario's model,
+// each with its own constants, in the reference's FP64 arithmetic, so every
+// scenario's S_T is the value its own path function would give.
+template <class Const>
+struct FdScenarios {
+  Const c[kMaxScen];
+  int n;
+  double strike;
+  double sign;  // payoff max(sign * (S_T - strike), 0): call +1, put -1 (greeks.rs:12-17)
+};
+
+template <class Model, int kMax>
+struct ModelSet {
+  Model m[kMax];
+  int n;
+  template <class Const, size_t... I>
+  __device__ __forceinline__ ModelSet(const Const* c, int n_, std::index_sequence<I...>)
+      : m{Model(c[I], false)...}, n(n_) {}
+  template <class... A>
+  __device__ __forceinline__ void apply(A... a) {
+#pragma unroll
+    for (int s = 0; s < kMax; ++s)
+      if (s < n) m[s].apply(a...);
+  }
+  template <class... A>
+  __device__ __forceinline__ void apply_jr(A... a) {
+#pragma unroll
+    for (int s = 0; s < kMax; ++s)
+      if (s < n) m[s].apply_jr(a...);
+  }
+};
+
+// Ring slots per lane for the FP64 sources: 8 normals, or 4 Merton entries
+// (16 bytes each), 16 KB per block either way.
+template <class Src>
+constexpr int kRing64 = sizeof(typename Src::Rec) > 8 ? 4 : 8;
+
+// kWaves: 3 for Heston (two FP64 state words per scenario), else 4
+template <class Src, class Model, class Const, int kQueue, int kWaves>
+__global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void
+fd_feed_kernel(const FdScenarios<Const> sc, const StreamDesc sd, const uint64_t n_paths,
+               const uint32_t steps, double* __restrict__ partials,
+               unsigned* __restrict__ counter, double* __restrict__ out) {
+  using Rec = typename Src::Rec;
+  __shared__ typename Src::Tables zig;
+  __shared__ Rec q[kQueue][kFusedBlock];
+  __shared__ double red[kFusedBlock / 64][kMaxAcc];
+  zig_load(&zig);
+  __syncthreads();
+
+  double acc[kMaxAcc];
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) acc[i] = 0.0;
+
+  const U128 inc = sd.inc;
+  const uint64_t stride = (uint64_t)gridDim.x * kFusedBlock;
+  for (uint64_t base = (uint64_t)blockIdx.x * kFusedBlock; base < n_paths; base += stride) {
+    const uint64_t p = base + threadIdx.x;
+    const bool valid = p < n_paths;
+    Pcg g{stream_at(&sd, valid ? p : 0)};
+    Src src(sc.c[0]);  // the draw grammar is the same in every scenario of the launch
+    ModelSet<Model, kMaxScen> ms(sc.c, sc.n, std::make_index_sequence<kMaxScen>());
+    queue_run<Src, kQueue, kFusedBlock>(src, ms, g, inc, steps, &zig, q);
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < kMaxScen; ++s) {
+        if (s < sc.n) {
+          double pay = sc.sign * (ms.m[s].terminal() - sc.strike);
+          acc[s] += pay > 0.0 ? pay : 0.0;
+        }
+      }
+    }
+  }
+  finish_sums(acc, sc.n, red, partials, counter, out);
+}
+
+template <class Model, class Src, class Const>
+static int run_fd(const Const* c, int n, double strike, double sign, const StreamDesc& sd,
+                  uint64_t n_paths, uint64_t steps, double* h_sums, hipStream_t st,
+                  const Err& err) {
+  if (steps > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps must be below 2^32");
+  if (n < 1 || n > kMaxScen) return err.set(STOCHA_EINVAL, "1 to %d scenarios per launch", kMaxScen);
+  FdScenarios<Const> sc{};
+  for (int i = 0; i < n; ++i) sc.c[i] = c[i];
+  for (int i = n; i < kMaxScen; ++i) sc.c[i] = c[0];
+  sc.n = n;
+  sc.strike = strike;
+  sc.sign = sign;
+  const unsigned grid = fused_grid(n_paths);
+  Workspace* ws = nullptr;
+  if (int rc = workspace(sizeof(double) * grid * n, sizeof(double) * kMaxAcc, &ws, err))
+    return rc;
+  KernelTimer timer(st);
+  timer.start();
+  constexpr int kWaves = std::is_same<Model, Heston64>::value ? 3 : 4;
+  fd_feed_kernel<Src, Model, Const, kRing64<Src>, kWaves><<<grid, kFusedBlock, 0, st>>>(
+      sc, sd, n_paths, (uint32_t)steps, static_cast<double*>(ws->dev), ws->counter, ws->host);
+  STOCHA_HIP(hipGetLastError());
+  timer.stop(1);
+  STOCHA_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < n; ++i) h_sums[i] = ws->host[i];
+  return STOCHA_OK;
+}
+
+int launch_fd(int model, const void* consts, int n, double strike, double sign,
+              const StreamDesc& sd, uint64_t n_paths, uint64_t steps, double* h_sums,
+              hipStream_t st, const Err& err) {
+  switch (model) {
+    case kGbm:
+      return run_fd<Gbm64, GbmSrc<false>>(static_cast<const GbmConst*>(consts), n, strike, sign,
+                                          sd, n_paths, steps, h_sums, st, err);
+    case kHeston:
+      return run_fd<Heston64, HestonSrc<false>>(static_cast<const HestonConst*>(consts), n,
+                                                strike, sign, sd, n_paths, steps, h_sums, st,
+                                                err);
+    case kMerton:
+      return run_fd<Merton64, MertonSrc<false, Merton64>>(static_cast<const MertonConst*>(consts),
+                                                          n, strike, sign, sd, n_paths, steps,
+                                                          h_sums, st, err);
+    default:
+      return err.set(STOCHA_EINVAL, "unknown model id %d", model);
+  }
+}
+
+// greeks.rs:91-167 (ModelSpec::terminal_prices): S_T of every path, FP64, in
+// the reference's arithmetic (the custom-payoff form of greeks_fd).
+template <class Src, class Model, class Const>
+__global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(4))) void
+terminals_kernel(const Const mc, const StreamDesc sd, const uint64_t n_paths,
+                 const uint32_t steps, double* __restrict__ out) {
+  using Rec = typename Src::Rec;
+  constexpr int kQueue = kRing64<Src>;
+  __shared__ typename Src::Tables zig;
+  __shared__ Rec q[kQueue][kFusedBlock];
+  zig_load(&zig);
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * kFusedBlock;
+  for (uint64_t base = (uint64_t)blockIdx.x * kFusedBlock; base < n_paths; base += stride) {
+    const uint64_t p = base + threadIdx.x;
+    const bool valid = p < n_paths;
+    Pcg g{stream_at(&sd, valid ? p : 0)};
+    const double st = queue_path<Src, Model, kQueue, kFusedBlock>(mc, g, sd.inc, steps, &zig, q);
+    if (valid) out[p] = st;
+  }
+}
+
+int launch_terminals(int model, const void* mconst, const StreamDesc& sd, uint64_t n_paths,
+                     uint64_t steps, double* d_out, hipStream_t st, const Err& err) {
+  if (steps > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps must be below 2^32");
+  if (n_paths == 0) return STOCHA_OK;
+  const unsigned grid = fused_grid(n_paths);
+  KernelTimer timer(st);
+  timer.start();
+  switch (model) {
+    case kGbm:
+      terminals_kernel<GbmSrc<false>, Gbm64><<<grid, kFusedBlock, 0, st>>>(
+          *static_cast<const GbmConst*>(mconst), sd, n_paths, (uint32_t)steps, d_out);
+      break;
+    case kHeston:
+      terminals_kernel<HestonSrc<false>, Heston64><<<grid, kFusedBlock, 0, st>>>(
+          *static_cast<const HestonConst*>(mconst), sd, n_paths, (uint32_t)steps, d_out);
+      break;
+    case kMerton:
+      terminals_kernel<MertonSrc<false, Merton64>, Merton64><<<grid, kFusedBlock, 0, st>>>(
+          *static_cast<const MertonConst*>(mconst), sd, n_paths, (uint32_t)steps, d_out);
+      break;
+    default:
+      return err.set(STOCHA_EINVAL, "unknown model id %d", model);
+  }
+  STOCHA_HIP(hipGetLastError());
+  timer.stop(1);
+  return STOCHA_OK;
 }
 
 }  // namespace stocha

@@ -176,6 +176,16 @@ enum Schedule : int { kScheduleFeed = 0, kScheduleLoop = 1 };
 int launch_fused(int model, int precision, int schedule, const void* mconst,
                  const StreamDesc& sd, uint64_t n_paths, uint64_t steps, const double* strikes,
                  int n_strikes, double* h_sums, hipStream_t st, const Err& err);
+// greeks_fd scenarios on shared draws: consts is an array of n (<= 8)
+// GbmConst / HestonConst / MertonConst whose draw grammar is the same (same
+// steps; for Merton the same lambda*dt); payoff max(sign*(S_T - strike), 0);
+// h_sums[i] = sum over paths of scenario i's payoff.
+int launch_fd(int model, const void* consts, int n, double strike, double sign,
+              const StreamDesc& sd, uint64_t n_paths, uint64_t steps, double* h_sums,
+              hipStream_t st, const Err& err);
+// S_T of paths 0..n_paths-1 into d_out (device), FP64 reference arithmetic.
+int launch_terminals(int model, const void* mconst, const StreamDesc& sd, uint64_t n_paths,
+                     uint64_t steps, double* d_out, hipStream_t st, const Err& err);
 // Fixed-order sum of partials[b][i] over b into out[i] (one block, on st).
 int launch_reduce_partials(const double* part, int n_blocks, int na, double* out, hipStream_t st,
                            const Err& err);
