@@ -223,6 +223,18 @@ def main():
             "ms_one_launch_per_scenario": t1 * 1e3, "speedup": t1 / tf,
             "values": S.greeks_fd(**fkw)}
 
+        # Heston QE (Andersen), 1,048,576 x 252 terminal prices, FP64: the
+        # variance recursion in the draw source, the log-price step per lane
+        from stocha_amd import _stocha as SS
+        qv = np.array([100.0, 0.04, 0.05, 2.0, 0.04, 0.3, -0.7, T])
+        tq = time_calls(lambda: SS._terminal_prices(1, qv, "qe", STEPS, N_PATHS, SEED), 3, 1)
+        kq = kernel_ms(S, lambda: SS._terminal_prices(1, qv, "qe", STEPS, N_PATHS, SEED))
+        te = time_calls(lambda: SS._terminal_prices(1, qv, "euler", STEPS, N_PATHS, SEED), 3, 1)
+        ke = kernel_ms(S, lambda: SS._terminal_prices(1, qv, "euler", STEPS, N_PATHS, SEED))
+        out["heston_qe_terminals_1Mx252"] = {
+            "ms": tq * 1e3, "kernel_ms": kq, "path_steps_per_s_kernel": ps / (kq * 1e-3),
+            "euler_same_api_ms": te * 1e3, "euler_kernel_ms": ke}
+
         # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
         from stocha_amd import batch as B
         inst = B.config4_instances(4096, seed=SEED)

@@ -7,7 +7,7 @@
  * validation order and ValueError texts, and its output layout:
  *
  *   stocha_gbm                   <- src/lib.rs:205-252   (gbm)
- *   stocha_heston                <- src/lib.rs:327-408   (heston, scheme="euler")
+ *   stocha_heston                <- src/lib.rs:327-408   (heston, both schemes)
  *   stocha_merton_jump_diffusion <- src/lib.rs:430-489   (merton_jump_diffusion)
  *   stocha_multi_gbm             <- src/lib.rs:759-827   (multi_gbm)
  *   stocha_lsmc_american_option  <- src/lib.rs:854-918   (lsmc_american_option)
@@ -96,10 +96,9 @@ int stocha_gbm(double s0, double mu, double sigma, double t, uint64_t steps, uin
                uint64_t seed, int antithetic, double q, double *out, int where, char *err,
                size_t errlen);
 
-/* src/lib.rs:327-408 -> heston.rs:32-87.  scheme: "euler" (full-truncation
- * Euler).  "qe" passes validation as in the reference but is not part of this
- * library's hot path: it returns STOCHA_ENOTIMPL with a message saying so.
- * out: (n_paths, steps+1) f64 row-major. */
+/* src/lib.rs:327-408 -> heston.rs:32-174.  scheme: "euler" (full-truncation
+ * Euler, heston.rs:44-87) or "qe" (Andersen's quadratic-exponential scheme,
+ * heston.rs:89-174); NULL = "euler".  out: (n_paths, steps+1) f64 row-major. */
 int stocha_heston(double s0, double v0, double mu, double kappa, double theta, double xi,
                   double rho, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
                   const char *scheme, double *out, int where, char *err, size_t errlen);
@@ -142,7 +141,7 @@ int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double s
  *   GBM    {s0, r, sigma, t}                          (n_params 4)
  *   Heston {s0, v0, r, kappa, theta, xi, rho, t}      (n_params 8)
  *   Merton {s0, r, sigma, lambda_, mu_j, sigma_j, t}  (n_params 7)
- * scheme (Heston only; NULL = "euler"; "qe" -> STOCHA_ENOTIMPL).  greeks:
+ * scheme (Heston only; NULL = "euler", or "qe").  greeks:
  * n_greeks names among delta/gamma/vega/theta/rho (checked first, lib.rs:920);
  * values[i] receives the i-th.  Bumps max(|p| * bump_size, 1e-8); every
  * scenario on the same seed (common random numbers); scenarios that draw

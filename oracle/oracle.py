@@ -45,6 +45,8 @@ def lib():
         L.orc_gbm.argtypes = [_d, _d, _d, _d, _u, _u, _u, _i, _d, _pd]
         L.orc_heston_euler.argtypes = [_d] * 8 + [_u, _u, _u, _pd]
         L.orc_merton.argtypes = [_d] * 7 + [_u, _u, _u, _pd]
+        L.orc_heston_qe.argtypes = [_d] * 8 + [_u, _u, _u, _pd]
+        L.orc_heston_qe_variance.argtypes = [_d] * 8 + [_u, _u, _u, _pd]
         L.orc_multi_gbm.argtypes = [_u, _pd, _pd, _pd, _pd, _d, _u, _u, _u, _i, _pd,
                                     C.c_char_p, C.c_size_t]
         L.orc_cholesky.argtypes = [_u, _pd, _pd, C.c_char_p, C.c_size_t]
@@ -114,6 +116,22 @@ def gbm(s0, mu, sigma, t, steps, n_paths, seed=42, antithetic=False, q=0.0):
 def heston(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed=42):
     out = np.empty((n_paths, steps + 1), np.float64)
     lib().orc_heston_euler(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed, _p(out))
+    return out
+
+
+def heston_qe(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed=42):
+    out = np.empty((n_paths, steps + 1), np.float64)
+    rc = lib().orc_heston_qe(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed,
+                             _p(out))
+    assert rc == 0
+    return out
+
+
+def heston_qe_variance(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed=42):
+    out = np.empty((n_paths, steps + 1), np.float64)
+    rc = lib().orc_heston_qe_variance(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths,
+                                      seed, _p(out))
+    assert rc == 0
     return out
 
 

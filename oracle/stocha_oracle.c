@@ -266,6 +266,91 @@ int orc_heston_euler(double s0, double v0, double mu, double kappa, double theta
 }
 
 /* ------------------------------------------------------------------------ */
+/* Heston QE (Andersen 2008): heston.rs:89-174, every expression per step as  */
+/* the reference writes it (no hoisting).                                     */
+/* ------------------------------------------------------------------------ */
+static double heston_qe_one(const heston_k *k, double t, uint64_t seed, uint64_t path,
+                            double *row, double *vrow) {
+    const double dt = t / (double)k->steps;               /* heston.rs:92 */
+    const double e_kdt = exp(-k->kappa * dt);              /* :93 */
+    const double k1 = k->kappa * k->theta;                 /* :94 */
+    pcg_t g;
+    pcg_for_path(&g, seed, k->block, path);                /* block :97 */
+    double s = k->s0, v = k->v0;
+    if (row) row[0] = s;
+    if (vrow) vrow[0] = v;
+    for (uint64_t j = 0; j < k->steps; j++) {
+        double v_plus = v > 0.0 ? v : 0.0;                 /* :109 (max, NaN -> 0) */
+        double m = k1 * (1.0 - e_kdt) / k->kappa + v_plus * e_kdt;   /* :112 */
+        m = m > 1e-15 ? m : 1e-15;                                    /* :113 */
+        double one_m = 1.0 - e_kdt;
+        double s2 = v_plus * k->xi * k->xi * e_kdt * (1.0 - e_kdt) / k->kappa
+                    + k1 * k->xi * k->xi * (one_m * one_m)
+                          / (2.0 * k->kappa * k->kappa);               /* :114-116 */
+        double psi = s2 / (m * m);                                    /* :117 */
+        double v_next;
+        if (psi <= 1.5) {                                             /* :119-126 */
+            double b2 = 2.0 / psi - 1.0 + sqrt(2.0 / psi * (2.0 / psi - 1.0));
+            b2 = b2 > 0.0 ? b2 : 0.0;
+            double a = m / (1.0 + b2);
+            double b = sqrt(b2);
+            double z = zig_normal(&g);
+            v_next = a * (b + z) * (b + z);
+        } else {                                                      /* :127-136 */
+            double p = (psi - 1.0) / (psi + 1.0);
+            double beta = (1.0 - p) / m;
+            double u = pcg_next_f64(&g);
+            v_next = u <= p ? 0.0 : log((1.0 - p) / (1.0 - u)) / beta;
+        }
+        v_next = v_next > 0.0 ? v_next : 0.0;                         /* :139 */
+        double z1 = zig_normal(&g);                                   /* :142 */
+        double kappa_rho_xi = k->kappa * k->rho / k->xi;              /* :143 */
+        double gamma1 = 0.5, gamma2 = 0.5;
+        double k0_s = (k->mu - kappa_rho_xi * k->theta) * dt;
+        double k1_s = gamma1 * dt * (kappa_rho_xi - 0.5) - k->rho / k->xi;
+        double k2_s = gamma2 * dt * (kappa_rho_xi - 0.5) + k->rho / k->xi;
+        double k3_s = gamma1 * dt * (1.0 - k->rho * k->rho);
+        double k4_s = gamma2 * dt * (1.0 - k->rho * k->rho);
+        double var_term = k3_s * v_plus + k4_s * v_next;              /* :152 */
+        var_term = var_term > 0.0 ? var_term : 0.0;
+        double log_s = log(s) + k0_s + k1_s * v_plus + k2_s * v_next + sqrt(var_term) * z1;
+        if (log_s < -500.0) log_s = -500.0;                           /* :155 clamp */
+        else if (log_s > 500.0) log_s = 500.0;
+        s = exp(log_s);
+        v = v_next;
+        if (row) row[j + 1] = s;
+        if (vrow) vrow[j + 1] = v;
+    }
+    return s;
+}
+
+/* The variance paths V(t_j) of the QE scheme (test hook: pins the variance
+ * construction by its conditional moments).  out: (n_paths, steps+1). */
+int orc_heston_qe_variance(double s0, double v0, double mu, double kappa, double theta,
+                           double xi, double rho, double t, uint64_t steps, uint64_t n_paths,
+                           uint64_t seed, double *out) {
+    double p[7] = {s0, v0, mu, kappa, theta, xi, rho};
+    heston_k k;
+    heston_k_init(&k, p, t, steps);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++)
+        heston_qe_one(&k, t, seed, (uint64_t)i, NULL, out + (uint64_t)i * (steps + 1));
+    return 0;
+}
+
+int orc_heston_qe(double s0, double v0, double mu, double kappa, double theta, double xi,
+                  double rho, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
+                  double *out) {
+    double p[7] = {s0, v0, mu, kappa, theta, xi, rho};
+    heston_k k;
+    heston_k_init(&k, p, t, steps);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++)
+        heston_qe_one(&k, t, seed, (uint64_t)i, out + (uint64_t)i * (steps + 1), NULL);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Merton jump-diffusion: jump_diffusion.rs:48-105                           */
 /* ------------------------------------------------------------------------ */
 typedef struct {
@@ -342,6 +427,12 @@ int orc_terminals(int model, const double *p, double t, uint64_t steps, uint64_t
 #pragma omp parallel for schedule(dynamic, 64)
         for (int64_t i = 0; i < (int64_t)n_paths; i++)
             out[i] = merton_one(&k, seed, (uint64_t)i, NULL);
+    } else if (model == 3) { /* Heston QE */
+        heston_k k;
+        heston_k_init(&k, p, t, steps);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t i = 0; i < (int64_t)n_paths; i++)
+            out[i] = heston_qe_one(&k, t, seed, (uint64_t)i, NULL, NULL);
     } else {
         return 1;
     }
@@ -685,7 +776,7 @@ enum { ORC_S0 = 0, ORC_SIGMA = 1, ORC_T = 2, ORC_MU = 3 };
 /* ModelSpec::param_value / with_bumped (greeks.rs:169-224): field index of
  * S0, Sigma (Heston: v0), T, Mu (the rate) */
 static int orc_fd_index(int model, int param) {
-    static const int idx[3][4] = {{0, 2, 3, 1}, {0, 1, 7, 2}, {0, 2, 6, 1}};
+    static const int idx[4][4] = {{0, 2, 3, 1}, {0, 1, 7, 2}, {0, 2, 6, 1}, {0, 1, 7, 2}};
     return idx[model][param];
 }
 
@@ -698,7 +789,7 @@ static double orc_mc_price(int model, const double *spec, double sign, double st
     if (model == 0) {
         p[0] = spec[0]; p[1] = spec[1]; p[2] = spec[2]; p[3] = 0.0;
         t = spec[3];
-    } else if (model == 1) {
+    } else if (model == 1 || model == 3) {
         for (int i = 0; i < 7; i++) p[i] = spec[i];
         t = spec[7];
     } else {
@@ -719,8 +810,8 @@ static double orc_mc_price(int model, const double *spec, double sign, double st
 int orc_greeks_fd(int model, const double *spec, double sign, double strike, uint64_t n_paths,
                   uint64_t steps, const int *greeks, int n_greeks, uint64_t seed,
                   double bump_size, double *values) {
-    if (model < 0 || model > 2) return 1;
-    const int nspec = model == 0 ? 4 : model == 1 ? 8 : 7;
+    if (model < 0 || model > 3) return 1;
+    const int nspec = model == 0 ? 4 : (model == 1 || model == 3) ? 8 : 7;
     /* required_scenarios (greeks.rs:231-265): key = param * 2 + (dir < 0), base = 8 */
     int keys[9], nk = 0;
 #define ORC_ADD(k) do { int _k = (k), _f = 0; for (int _i = 0; _i < nk; _i++) _f |= keys[_i] == _k; \

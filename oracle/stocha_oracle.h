@@ -40,6 +40,14 @@ int orc_gbm(double s0, double mu, double sigma, double t, uint64_t steps, uint64
 int orc_heston_euler(double s0, double v0, double mu, double kappa, double theta, double xi,
                      double rho, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
                      double *out);
+/* heston.rs:89-174 (QE); same layout.  _variance: the V(t_j) paths instead
+ * of S (a test hook for the variance scheme's conditional moments). */
+int orc_heston_qe_variance(double s0, double v0, double mu, double kappa, double theta,
+                           double xi, double rho, double t, uint64_t steps, uint64_t n_paths,
+                           uint64_t seed, double *out);
+int orc_heston_qe(double s0, double v0, double mu, double kappa, double theta, double xi,
+                  double rho, double t, uint64_t steps, uint64_t n_paths, uint64_t seed,
+                  double *out);
 int orc_merton(double s0, double mu, double sigma, double lambda, double mu_j, double sigma_j,
                double t, uint64_t steps, uint64_t n_paths, uint64_t seed, double *out);
 /* out is (n_paths, steps+1, d) row-major */
@@ -54,7 +62,7 @@ int orc_multi_terminals(uint64_t d, const double *s0, const double *mu, const do
 
 /* ---- terminal-only steppers (same arithmetic as above, last column only) ---- */
 /* model: 0 GBM (p = s0, mu, sigma, q), 1 Heston (s0, v0, mu, kappa, theta, xi, rho),
- *        2 Merton (s0, mu, sigma, lambda, mu_j, sigma_j) */
+ *        2 Merton (s0, mu, sigma, lambda, mu_j, sigma_j), 3 Heston QE (as 1) */
 int orc_terminals(int model, const double *p, double t, uint64_t steps, uint64_t n_paths,
                   uint64_t seed, double *out);
 /* European calls on terminals: sum[k] = sum_i max(S_T,i - K_k, 0), sumsq likewise,
@@ -82,7 +90,8 @@ int orc_greeks_pathwise(double s0, double r, double sigma, double t, double stri
 
 /* ---- bump-and-revalue Greeks (greeks.rs:227-358) ----
  * spec in the reference's params-dict order per model (GBM 4, Heston 8,
- * Merton 7 values); sign +1 call, -1 put; greeks ids 0..4 = delta, gamma,
+ * Merton 7 values; model 3 = Heston QE with Heston's spec); sign +1 call,
+ * -1 put; greeks ids 0..4 = delta, gamma,
  * vega, theta, rho.  Prices with left-to-right payoff sums (mc_price). */
 int orc_greeks_fd(int model, const double *spec, double sign, double strike, uint64_t n_paths,
                   uint64_t steps, const int *greeks, int n_greeks, uint64_t seed,

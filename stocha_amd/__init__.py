@@ -5,12 +5,12 @@ A drop-in for these functions of the reference's Python API
 errors:
 
     gbm                    (__init__.py:180-231)
-    heston                 (__init__.py:276-324; scheme "euler")
+    heston                 (__init__.py:276-324; schemes "euler" and "qe")
     merton_jump_diffusion  (__init__.py:327-373)
     multi_gbm              (__init__.py:574-627)
     lsmc_american_option   (__init__.py:630-671)
     greeks_pathwise        (__init__.py:725-769)
-    greeks_fd              (__init__.py:674-722; Heston scheme "euler")
+    greeks_fd              (__init__.py:674-722)
 
 They run on the GPU through the C ABI of libstocha_gpu.so (hand-written HIP
 kernels); there is no CPU fallback.  Forms beside the drop-in API:
@@ -66,8 +66,8 @@ def gbm(s0: float, mu: float, sigma: float, t: float, steps: int, n_paths: int,
 def heston(s0: float, v0: float, mu: float, kappa: float, theta: float, xi: float, rho: float,
            t: float, steps: int, n_paths: int, seed: int = 42,
            scheme: Literal["euler", "qe"] = "euler") -> np.ndarray:
-    """Simulate Heston paths, full-truncation Euler (reference:
-    python/stocha/__init__.py:276-324).  scheme="qe" raises NotImplementedError."""
+    """Simulate Heston paths, full-truncation Euler or Andersen's QE scheme
+    (reference: python/stocha/__init__.py:276-324)."""
     return _S.heston(s0=s0, v0=v0, mu=mu, kappa=kappa, theta=theta, xi=xi, rho=rho, t=t,
                      steps=steps, n_paths=n_paths, seed=seed, scheme=scheme)
 

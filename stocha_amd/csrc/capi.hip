@@ -94,6 +94,32 @@ HestonConst heston_const(const double* p, double t, uint64_t steps) {
   return c;
 }
 
+// heston.rs:91-97 and the loop-invariant parts of :112-150, same operation
+// order as the reference's per-step expressions.
+HestonQeConst heston_qe_const(const double* p, double t, uint64_t steps) {
+  HestonQeConst c;
+  const double mu = p[2], kappa = p[3], theta = p[4], xi = p[5], rho = p[6];
+  const double dt = t / (double)steps;     // :92
+  const double e_kdt = exp(-kappa * dt);   // :93
+  const double k1 = kappa * theta;         // :94
+  c.s0 = p[0];
+  c.v0 = p[1];
+  c.kappa = kappa;
+  c.xi = xi;
+  c.e_kdt = e_kdt;
+  c.one_m_e = 1.0 - e_kdt;
+  c.m_const = k1 * (1.0 - e_kdt) / kappa;  // :112
+  c.s2_const = k1 * xi * xi * ((1.0 - e_kdt) * (1.0 - e_kdt)) / (2.0 * kappa * kappa);  // :115-116
+  const double kappa_rho_xi = kappa * rho / xi;  // :143
+  const double gamma1 = 0.5, gamma2 = 0.5;
+  c.k0s = (mu - kappa_rho_xi * theta) * dt;                  // :146
+  c.k1s = gamma1 * dt * (kappa_rho_xi - 0.5) - rho / xi;     // :147
+  c.k2s = gamma2 * dt * (kappa_rho_xi - 0.5) + rho / xi;     // :148
+  c.k3s = gamma1 * dt * (1.0 - rho * rho);                   // :149
+  c.k4s = gamma2 * dt * (1.0 - rho * rho);                   // :150
+  return c;
+}
+
 MertonConst merton_const(const double* p, double t, uint64_t steps) {
   MertonConst c;
   const double s0 = p[0], mu = p[1], sigma = p[2], lambda = p[3], mu_j = p[4], sigma_j = p[5];
@@ -236,25 +262,51 @@ double fd_bump(double v, double bump_size) {
 }
 
 // The spec's model constants (ModelSpec::terminal_prices, greeks.rs:91-167;
-// GBM with q = 0 and no antithetic paths).
-void fd_consts(int model, const double* v, uint64_t steps, GbmConst* g, HestonConst* h,
-               MertonConst* m) {
-  if (model == kGbm) {
-    *g = gbm_const(v[0], v[1], v[2], v[3], steps, 0.0);
-  } else if (model == kHeston) {
-    *h = heston_const(v, v[7], steps);
-  } else {
-    *m = merton_const(v, v[6], steps);
+// GBM with q = 0 and no antithetic paths).  kind: kGbm, kHeston (Euler),
+// kMerton or kHestonQe.
+struct FdConsts {
+  GbmConst g;
+  HestonConst h;
+  MertonConst m;
+  HestonQeConst q;
+  const void* ptr(int kind) const {
+    return kind == kGbm ? (const void*)&g : kind == kHeston ? (const void*)&h
+                                          : kind == kMerton ? (const void*)&m : (const void*)&q;
   }
+};
+
+FdConsts fd_consts(int kind, const double* v, uint64_t steps) {
+  FdConsts c{};
+  if (kind == kGbm) c.g = gbm_const(v[0], v[1], v[2], v[3], steps, 0.0);
+  else if (kind == kHeston) c.h = heston_const(v, v[7], steps);
+  else if (kind == kHestonQe) c.q = heston_qe_const(v, v[7], steps);
+  else c.m = merton_const(v, v[6], steps);
+  return c;
 }
 
-int fd_check_scheme(int model, const char* scheme, const Err& err) {
-  if (model != kHeston) return STOCHA_OK;  // only the Heston spec reads it (lib.rs:988-1000)
+// Whether two scenarios consume the same draws (one launch can serve both):
+// GBM and Heston Euler always (normals only); Merton when lambda*dt's jump
+// threshold is the same; Heston QE when the variance recursion is the same
+// (its kind of first draw per step follows the variance path).
+bool fd_same_draws(int kind, const FdConsts& a, const FdConsts& b) {
+  if (kind == kMerton) return a.m.jump_t53 == b.m.jump_t53;
+  if (kind == kHestonQe)
+    return a.q.v0 == b.q.v0 && a.q.kappa == b.q.kappa && a.q.xi == b.q.xi &&
+           a.q.e_kdt == b.q.e_kdt && a.q.m_const == b.q.m_const && a.q.s2_const == b.q.s2_const;
+  return true;
+}
+
+// Heston's scheme (lib.rs:988-1000; only the Heston spec reads it): sets
+// *kind to kHeston or kHestonQe.
+int fd_kind(int model, const char* scheme, int* kind, const Err& err) {
+  *kind = model;
+  if (model != kHeston) return STOCHA_OK;
   const char* sch = scheme ? scheme : "euler";
   if (strcmp(sch, "euler") == 0) return STOCHA_OK;
-  if (strcmp(sch, "qe") == 0)
-    return err.set(STOCHA_ENOTIMPL,
-                   "scheme 'qe' is not part of the GPU hot path (only 'euler' is implemented)");
+  if (strcmp(sch, "qe") == 0) {
+    *kind = kHestonQe;
+    return STOCHA_OK;
+  }
   return err.set(STOCHA_EINVAL, "scheme must be 'euler' or 'qe'");
 }
 
@@ -330,15 +382,16 @@ int stocha_heston(double s0, double v0, double mu, double kappa, double theta, d
   if (steps == 0 || n_paths == 0)
     return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
   const char* sch = scheme ? scheme : "euler";
-  if (strcmp(sch, "euler") != 0) {
-    if (strcmp(sch, "qe") == 0)
-      return err.set(STOCHA_ENOTIMPL,
-                     "scheme 'qe' is not part of the GPU hot path (only 'euler' is implemented)");
-    return err.set(STOCHA_EINVAL, "scheme must be 'euler' or 'qe'");
-  }
+  const bool qe = strcmp(sch, "qe") == 0;
+  if (!qe && strcmp(sch, "euler") != 0) return err.set(STOCHA_EINVAL, "scheme must be 'euler' or 'qe'");
   if (!out) return STOCHA_OK;  // validate only
   std::lock_guard<std::mutex> lk(globals().mu);
   const double p[7] = {s0, v0, mu, kappa, theta, xi, rho};
+  if (qe) {  // heston.rs:89-174
+    const HestonQeConst c = heston_qe_const(p, t, steps);
+    return store_paths(kHestonQe, &c, seed, block_size(kHestonQe, steps), n_paths, n_paths, steps,
+                       out, where, err);
+  }
   HestonConst c = heston_const(p, t, steps);
   return store_paths(kHeston, &c, seed, block_size(kHeston, steps), n_paths, n_paths, steps, out,
                      where, err);
@@ -501,7 +554,8 @@ int stocha_greeks_fd(int model, const double* params, uint64_t n_params, const c
   if (n_params != kFdParams[model])
     return err.set(STOCHA_EINVAL, "model %d takes %llu parameters, got %llu", model,
                    (unsigned long long)kFdParams[model], (unsigned long long)n_params);
-  if (int rc = fd_check_scheme(model, scheme, err)) return rc;
+  int kind;
+  if (int rc = fd_kind(model, scheme, &kind, err)) return rc;
   double sign;
   if (payoff && strcmp(payoff, "call") == 0) {
     sign = 1.0;
@@ -529,34 +583,35 @@ int stocha_greeks_fd(int model, const double* params, uint64_t n_params, const c
   std::vector<std::vector<double>> spec(ns, std::vector<double>(params, params + n_params));
   for (int i = 0; i < ns; ++i)
     if (sc[i].param >= 0) spec[i][fd_index(model, sc[i].param)] += sc[i].dir * bump[sc[i].param];
-  std::vector<GbmConst> gc(ns);
-  std::vector<HestonConst> hc(ns);
-  std::vector<MertonConst> mc(ns);
-  for (int i = 0; i < ns; ++i) fd_consts(model, spec[i].data(), n_steps, &gc[i], &hc[i], &mc[i]);
+  std::vector<FdConsts> cs(ns);
+  for (int i = 0; i < ns; ++i) cs[i] = fd_consts(kind, spec[i].data(), n_steps);
 
-  // one launch per group of scenarios with the same draw grammar: all of
-  // them for GBM / Heston; for Merton those with the same lambda*dt threshold
+  // one launch per group of scenarios that consume the same draws
   StreamDesc sd;
-  make_stream_desc(seed, block_size(model, n_steps), &sd);
+  make_stream_desc(seed, block_size(kind, n_steps), &sd);
   std::vector<double> sum(ns, 0.0);
   std::vector<bool> done(ns, false);
   for (int i = 0; i < ns; ++i) {
     if (done[i]) continue;
     std::vector<int> grp;
     for (int j = i; j < ns; ++j)
-      if (!done[j] && (model != kMerton || mc[j].jump_t53 == mc[i].jump_t53)) grp.push_back(j);
+      if (!done[j] && fd_same_draws(kind, cs[i], cs[j])) grp.push_back(j);
     std::vector<GbmConst> g2;
     std::vector<HestonConst> h2;
     std::vector<MertonConst> m2;
+    std::vector<HestonQeConst> q2;
     for (int j : grp) {
-      g2.push_back(gc[j]);
-      h2.push_back(hc[j]);
-      m2.push_back(mc[j]);
+      g2.push_back(cs[j].g);
+      h2.push_back(cs[j].h);
+      m2.push_back(cs[j].m);
+      q2.push_back(cs[j].q);
     }
-    const void* cp = model == kGbm ? (const void*)g2.data()
-                     : model == kHeston ? (const void*)h2.data() : (const void*)m2.data();
+    const void* cp = kind == kGbm      ? (const void*)g2.data()
+                     : kind == kHeston ? (const void*)h2.data()
+                     : kind == kMerton ? (const void*)m2.data()
+                                       : (const void*)q2.data();
     double h[8];
-    if (int rc = launch_fd(model, cp, (int)grp.size(), strike, sign, sd, n_paths, n_steps, h,
+    if (int rc = launch_fd(kind, cp, (int)grp.size(), strike, sign, sd, n_paths, n_steps, h,
                            globals().stream, err))
       return rc;
     for (size_t k = 0; k < grp.size(); ++k) {
@@ -603,27 +658,24 @@ int stocha_terminal_prices(int model, const double* params, uint64_t n_params, c
   if (n_params != kFdParams[model])
     return err.set(STOCHA_EINVAL, "model %d takes %llu parameters, got %llu", model,
                    (unsigned long long)kFdParams[model], (unsigned long long)n_params);
-  if (int rc = fd_check_scheme(model, scheme, err)) return rc;
+  int kind;
+  if (int rc = fd_kind(model, scheme, &kind, err)) return rc;
   if (!out || n_paths == 0) return STOCHA_OK;
   std::lock_guard<std::mutex> lk(globals().mu);
   if (int rc = check_device(err)) return rc;
-  GbmConst g;
-  HestonConst h;
-  MertonConst m;
-  fd_consts(model, params, n_steps, &g, &h, &m);
-  const void* cp = model == kGbm ? (const void*)&g : model == kHeston ? (const void*)&h
-                                                                      : (const void*)&m;
+  const FdConsts c = fd_consts(kind, params, n_steps);
+  const void* cp = c.ptr(kind);
   StreamDesc sd;
-  make_stream_desc(seed, block_size(model, n_steps), &sd);
+  make_stream_desc(seed, block_size(kind, n_steps), &sd);
   hipStream_t st = globals().stream;
   if (where == STOCHA_DEVICE) {
-    if (int rc = launch_terminals(model, cp, sd, n_paths, n_steps, out, st, err)) return rc;
+    if (int rc = launch_terminals(kind, cp, sd, n_paths, n_steps, out, st, err)) return rc;
     STOCHA_HIP(hipStreamSynchronize(st));
     return STOCHA_OK;
   }
   Workspace* ws = nullptr;
   if (int rc = workspace(sizeof(double) * n_paths, 0, &ws, err)) return rc;
-  if (int rc = launch_terminals(model, cp, sd, n_paths, n_steps, static_cast<double*>(ws->dev),
+  if (int rc = launch_terminals(kind, cp, sd, n_paths, n_steps, static_cast<double*>(ws->dev),
                                 st, err))
     return rc;
   STOCHA_HIP(hipMemcpyAsync(out, ws->dev, sizeof(double) * n_paths, hipMemcpyDeviceToHost, st));

@@ -15,7 +15,7 @@ namespace stocha {
 // feed(u, z, rec, active) consumes u only when `active` (state unchanged
 // otherwise) and returns true when rec holds a new entry.
 //
-// GBM (gbm.rs:60-66) and Heston (heston.rs:92-107) take only normals, so the
+// GBM (gbm.rs:60-66) and Heston Euler (heston.rs:59-71) take only normals, so the
 // entries are single normals: 1 per GBM step, 2 per Heston step (dw1 first,
 // then the independent dz).
 template <bool kFp32, uint32_t kPer>
@@ -90,6 +90,83 @@ struct MertonSrc {
   __device__ static __forceinline__ void apply(MM& m, const Rec (*q)[kCols], uint32_t head) {
     const Rec r = q[head][threadIdx.x];
     m.apply_jr(r.z, r.jr);
+  }
+};
+
+// Heston QE (heston.rs:108-159): a step draws the variance sample -- a normal
+// when psi <= 1.5, else one uniform -- and then the asset normal z1.  Which
+// kind the first draw is depends on V(t), so the source runs the variance
+// recursion itself (it needs only the draws) and queues (V(t+dt), z1) per
+// step; the model does the log-price update.  FP64, the reference's
+// arithmetic; the per-step setup (m, s2, psi and the branch constants for
+// the next variance draw) runs when a variance sample completes.
+template <class M>
+struct HestonQeSrc {
+  using Real = double;
+  using Tables = ZigLds;
+  struct Rec {
+    double vn, z1;
+  };
+  static constexpr uint32_t kPerStep = 1;
+  static constexpr uint32_t kDefer = 0;
+  static constexpr int kWaves = 4;
+  ZigFeed<false> zf;
+  uint32_t phase = 0;  // 0: variance draw next, 1: asset normal next
+  bool quad = true;    // the pending variance draw is a normal (quadratic branch)
+  double qa = 0, qb = 0;  // quadratic: a, b; exponential: p, beta
+  double vn = 0;          // V(t+dt) of the step being assembled
+  HestonQeConst c;
+  template <class C>
+  __device__ __forceinline__ explicit HestonQeSrc(const C& k) : c(k) {
+    setup(k.v0);
+  }
+  // heston.rs:109-130: the branch for the variance draw out of V(t) = v
+  __device__ __forceinline__ void setup(double v) {
+    const double v_plus = fmax(v, 0.0);
+    double m = c.m_const + v_plus * c.e_kdt;
+    m = fmax(m, 1e-15);
+    const double s2 = v_plus * c.xi * c.xi * c.e_kdt * c.one_m_e / c.kappa + c.s2_const;
+    const double psi = s2 / (m * m);
+    quad = psi <= 1.5;
+    if (quad) {
+      const double two_psi = 2.0 / psi;
+      double b2 = two_psi - 1.0 + sqrt(two_psi * (two_psi - 1.0));
+      b2 = fmax(b2, 0.0);
+      qa = m / (1.0 + b2);
+      qb = sqrt(b2);
+    } else {
+      const double p = (psi - 1.0) / (psi + 1.0);
+      qa = p;
+      qb = (1.0 - p) / m;
+    }
+  }
+  __device__ __forceinline__ bool pending() const { return zf.pending(); }
+  __device__ __forceinline__ bool feed(uint64_t u, const Tables* z, Rec& r, bool active) {
+    double x;
+    const bool uni = active & (phase == 0u) & !quad;
+    const bool got = zf.feed(u, z, x, active & !uni);
+    const bool var_done = uni | (got & (phase == 0u));
+    const bool out = got & (phase == 1u);
+    r.vn = vn;
+    r.z1 = x;
+    if (var_done) {
+      double v;
+      if (uni) {
+        const double uf = (double)(u >> 11) * 0x1p-53;  // next_f64 (pcg64.rs:37-39)
+        v = uf <= qa ? 0.0 : log((1.0 - qa) / (1.0 - uf)) / qb;
+      } else {
+        v = qa * (qb + x) * (qb + x);
+      }
+      vn = fmax(v, 0.0);
+      setup(vn);  // the next step's branch
+    }
+    phase = var_done ? 1u : (out ? 0u : phase);
+    return out;
+  }
+  template <uint32_t kSlots, int kCols, class MM>
+  __device__ static __forceinline__ void apply(MM& m, const Rec (*q)[kCols], uint32_t head) {
+    const Rec r = q[head][threadIdx.x];
+    m.apply_qe(r.vn, r.z1);
   }
 };
 

@@ -316,7 +316,9 @@ int run_greeks_pathwise(const GbmConst& gc, const StreamDesc& sd, uint64_t n_pat
 // scenario of greeks_fd_core prices the same paths with the same seed and
 // block size (common random numbers), and a bump of s0, sigma / v0, mu or t
 // leaves the sequence of draws a path consumes unchanged -- except Merton's t
-// bump, which moves lambda*dt and so which uniforms jump.  One launch
+// bump, which moves lambda*dt and so which uniforms jump, and, under the
+// Heston QE scheme, the v0 and t bumps, which move the variance path and so
+// whether a step's first draw is a normal or a uniform.  One launch
 // therefore steps up to kMaxScen scenarios of a path on one stream of draws:
 // the queue entries are produced once and applied to every scenario's model,
 // each with its own constants, in the reference's FP64 arithmetic, so every
@@ -347,6 +349,12 @@ struct ModelSet {
 #pragma unroll
     for (int s = 0; s < kMax; ++s)
       if (s < n) m[s].apply_jr(a...);
+  }
+  template <class... A>
+  __device__ __forceinline__ void apply_qe(A... a) {
+#pragma unroll
+    for (int s = 0; s < kMax; ++s)
+      if (s < n) m[s].apply_qe(a...);
   }
 };
 
@@ -412,7 +420,8 @@ static int run_fd(const Const* c, int n, double strike, double sign, const Strea
     return rc;
   KernelTimer timer(st);
   timer.start();
-  constexpr int kWaves = std::is_same<Model, Heston64>::value ? 3 : 4;
+  constexpr int kWaves =
+      std::is_same<Model, Heston64>::value || std::is_same<Model, HestonQe64>::value ? 3 : 4;
   fd_feed_kernel<Src, Model, Const, kRing64<Src>, kWaves><<<grid, kFusedBlock, 0, st>>>(
       sc, sd, n_paths, (uint32_t)steps, static_cast<double*>(ws->dev), ws->counter, ws->host);
   STOCHA_HIP(hipGetLastError());
@@ -437,6 +446,10 @@ int launch_fd(int model, const void* consts, int n, double strike, double sign,
       return run_fd<Merton64, MertonSrc<false, Merton64>>(static_cast<const MertonConst*>(consts),
                                                           n, strike, sign, sd, n_paths, steps,
                                                           h_sums, st, err);
+    case kHestonQe:
+      return run_fd<HestonQe64, HestonQeSrc<HestonQe64>>(
+          static_cast<const HestonQeConst*>(consts), n, strike, sign, sd, n_paths, steps, h_sums,
+          st, err);
     default:
       return err.set(STOCHA_EINVAL, "unknown model id %d", model);
   }
@@ -483,6 +496,10 @@ int launch_terminals(int model, const void* mconst, const StreamDesc& sd, uint64
     case kMerton:
       terminals_kernel<MertonSrc<false, Merton64>, Merton64><<<grid, kFusedBlock, 0, st>>>(
           *static_cast<const MertonConst*>(mconst), sd, n_paths, (uint32_t)steps, d_out);
+      break;
+    case kHestonQe:
+      terminals_kernel<HestonQeSrc<HestonQe64>, HestonQe64><<<grid, kFusedBlock, 0, st>>>(
+          *static_cast<const HestonQeConst*>(mconst), sd, n_paths, (uint32_t)steps, d_out);
       break;
     default:
       return err.set(STOCHA_EINVAL, "unknown model id %d", model);

@@ -164,6 +164,7 @@ inline void make_stream_desc(uint64_t seed, uint64_t block, StreamDesc* d) {
 // binary64 (this file is compiled with -ffp-contract=off).
 GbmConst gbm_const(double s0, double mu, double sigma, double t, uint64_t steps, double q);
 HestonConst heston_const(const double* p7, double t, uint64_t steps);
+HestonQeConst heston_qe_const(const double* p7, double t, uint64_t steps);
 MertonConst merton_const(const double* p6, double t, uint64_t steps);
 
 // ------------------------------------------------------------ launchers --

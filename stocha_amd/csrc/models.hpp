@@ -25,7 +25,7 @@
 
 namespace stocha {
 
-enum ModelKind : int { kGbm = 0, kHeston = 1, kMerton = 2 };
+enum ModelKind : int { kGbm = 0, kHeston = 1, kMerton = 2, kHestonQe = 3 };
 
 struct GbmConst {
   double s0, drift, diffusion;
@@ -47,9 +47,24 @@ struct MertonConst {
   float drift_f, diffusion_f, mu_j_f, sigma_j_f;
 };
 
-// Stream budget per path (reference block sizes).
+// Heston QE (heston.rs:91-166).  The loop-invariant parts of the
+// reference's per-step expressions, evaluated once on the host with the same
+// operations in the same order.
+struct HestonQeConst {
+  double s0, v0, kappa, xi;
+  double e_kdt;       // exp(-kappa dt)
+  double one_m_e;     // 1 - e_kdt
+  double m_const;     // k1 * (1 - e_kdt) / kappa, k1 = kappa theta
+  double s2_const;    // k1 xi xi (1 - e_kdt)^2 / (2 kappa kappa)
+  double k0s, k1s, k2s, k3s, k4s;  // Andersen's log-price coefficients
+};
+
+// Stream budget per path (reference block sizes; QE keeps Euler's,
+// heston.rs:97).
 __host__ __device__ inline uint64_t block_size(int model, uint64_t steps) {
-  return model == kGbm ? (steps + 1024) * 2 : model == kHeston ? (steps + 1024) * 6 : (steps + 1024) * 8;
+  return model == kGbm                            ? (steps + 1024) * 2
+         : model == kHeston || model == kHestonQe ? (steps + 1024) * 6
+                                                  : (steps + 1024) * 8;
 }
 
 // ---------------------------------------------------------------- FP64 ----
@@ -88,6 +103,27 @@ struct Heston64 {
     double dz = zig_normal64(g, inc, z);
     apply(dw1, dz);
     return s;
+  }
+  __device__ __forceinline__ double terminal() const { return s; }
+};
+
+// Heston QE, log-price part (heston.rs:152-157); the variance recursion runs
+// in the draw source (HestonQeSrc), which hands over V(t+dt) with the
+// step's asset normal.  v tracks the same recursion (v0, then each V(t+dt)).
+struct HestonQe64 {
+  static constexpr bool kFp32 = false;
+  HestonQeConst c;
+  double s, v;
+  __device__ __forceinline__ HestonQe64(const HestonQeConst& k, bool) : c(k), s(k.s0), v(k.v0) {}
+  __device__ __forceinline__ void apply_qe(double v_next, double z1) {
+    const double v_plus = fmax(v, 0.0);
+    double var_term = c.k3s * v_plus + c.k4s * v_next;
+    var_term = fmax(var_term, 0.0);
+    double log_s = log(s) + c.k0s + c.k1s * v_plus + c.k2s * v_next + sqrt(var_term) * z1;
+    // f64::clamp(-500, 500) keeps a NaN
+    log_s = log_s < -500.0 ? -500.0 : (log_s > 500.0 ? 500.0 : log_s);
+    s = exp(log_s);
+    v = v_next;
   }
   __device__ __forceinline__ double terminal() const { return s; }
 };

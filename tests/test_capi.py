@@ -97,9 +97,12 @@ def test_heston_validation(bad, msg):
         S.heston(**{**HESTON, **bad})
 
 
-def test_heston_qe_is_outside_the_gpu_path():
-    with pytest.raises(NotImplementedError, match="qe"):
-        S.heston(**HESTON, scheme="qe")
+def test_heston_qe_validates_like_euler():
+    # scheme "qe" takes the same checks, in the same order (lib.rs:344-392)
+    with pytest.raises(ValueError, match="rho must be in"):
+        S.heston(**{**HESTON, "rho": 2.0}, scheme="qe")
+    with pytest.raises(ValueError, match="scheme must be 'euler' or 'qe'"):
+        S.heston(**HESTON, scheme="QE")
 
 
 MERTON = dict(s0=100.0, mu=0.05, sigma=0.2, lambda_=1.0, mu_j=-0.05, sigma_j=0.1, t=1.0,
