@@ -235,6 +235,25 @@ def main():
             "ms": tq * 1e3, "kernel_ms": kq, "path_steps_per_s_kernel": ps / (kq * 1e-3),
             "euler_same_api_ms": te * 1e3, "euler_kernel_ms": ke}
 
+        # path-dependent Monte Carlo on GBM, 1,048,576 x 252, FP64 (method="mc"):
+        # up-and-out call with a rebate at the hit, floating lookback call,
+        # arithmetic Asian call with the geometric control variate
+        ex = {}
+        ekw = dict(s=100.0, r=R, sigma=0.2, t=T, n_paths=N_PATHS, seed=SEED, method="mc")
+        cases = {
+            "barrier_uo_call_rebate": lambda: S.barrier_price(k=100.0, barrier=120.0,
+                                                              n_steps=STEPS, rebate=3.0,
+                                                              rebate_at_hit=True, **ekw),
+            "lookback_floating_call": lambda: S.lookback_price(n_steps=STEPS, **ekw),
+            "asian_arithmetic_call_cv": lambda: S.asian_price(k=100.0, n_steps=STEPS, **ekw),
+        }
+        for name, fn in cases.items():
+            te = time_calls(fn, 3, 1)
+            ke = kernel_ms(S, fn)
+            ex[name] = {"ms": te * 1e3, "kernel_ms": ke, "price": fn(),
+                        "path_steps_per_s": ps / te}
+        out["exotics_mc_1Mx252"] = ex
+
         # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
         from stocha_amd import batch as B
         inst = B.config4_instances(4096, seed=SEED)

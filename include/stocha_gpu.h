@@ -14,6 +14,9 @@
  *   stocha_greeks_pathwise       <- src/lib.rs:1177-1242 (greeks_pathwise)
  *   stocha_greeks_fd             <- src/lib.rs:1064-1156 (greeks_fd, built-in payoffs)
  *   stocha_terminal_prices       <- src/finance/greeks.rs:91-167 (greeks_fd, callable payoffs)
+ *   stocha_barrier_price         <- src/lib.rs:1551-1638 (barrier_price)
+ *   stocha_asian_price           <- src/lib.rs:1647-1745 (asian_price)
+ *   stocha_lookback_price        <- src/lib.rs:1761-1848 (lookback_price)
  *
  * Conventions (all entry points):
  *   - Plain C types only.  Integer arguments the reference takes as usize/u64
@@ -160,6 +163,36 @@ int stocha_greeks_fd(int model, const double *params, uint64_t n_params, const c
 int stocha_terminal_prices(int model, const double *params, uint64_t n_params, const char *scheme,
                            uint64_t n_steps, uint64_t n_paths, uint64_t seed, double *out,
                            int where, char *err, size_t errlen);
+
+/* ---- path-dependent options on GBM (barrier.rs, asian.rs, lookback.rs) ----
+ * Same validation, dispatch and message texts as the #[pyfunction]s.  method
+ * "auto" / "analytical" evaluate the reference's closed forms on the host
+ * (Reiner-Rubinstein with the Broadie-Glasserman-Kou shift and rebates;
+ * Kemna-Vorst; Goldman-Sosin-Gatto and Conze-Viswanathan); "mc" -- and
+ * "auto" where no finite closed form exists -- simulates on the device on
+ * gbm_paths' block-split streams (block (n_steps + 1024) * 2).  Optional
+ * arguments of the reference (Option<...>) come as a has_ flag and a value.
+ * price receives the result. */
+/* src/lib.rs:1551-1638 -> barrier.rs:145-372 */
+int stocha_barrier_price(double s, double k, double r, double sigma, double t, double barrier,
+                         const char *barrier_type, const char *option_type, double q,
+                         uint64_t n_paths, uint64_t n_steps, uint64_t seed, const char *method,
+                         int has_n_monitoring, uint32_t n_monitoring, double rebate,
+                         int rebate_at_hit, double *price, char *err, size_t errlen);
+/* src/lib.rs:1647-1745 -> asian.rs:78-355 (arithmetic fixed strike: geometric
+ * control variate, per-path pairs reduced in two fixed-order passes) */
+int stocha_asian_price(double s, double k, double r, double sigma, double t, uint64_t n_steps,
+                       const char *average_type, const char *strike_type,
+                       const char *option_type, double q, uint64_t n_paths, uint64_t seed,
+                       const char *method, int has_running_avg, double running_avg,
+                       int has_time_elapsed, double time_elapsed, double *price, char *err,
+                       size_t errlen);
+/* src/lib.rs:1761-1848 -> lookback.rs:68-210 */
+int stocha_lookback_price(double s, double r, double sigma, double t, uint64_t n_steps,
+                          const char *strike_type, const char *option_type, double k, double q,
+                          uint64_t n_paths, uint64_t seed, const char *method,
+                          int has_running_max, double running_max, int has_running_min,
+                          double running_min, double *price, char *err, size_t errlen);
 
 /* ---- forms beside the drop-in (no reference counterpart; they extend the
  *      same path and use the same streams) ------------------------------- */

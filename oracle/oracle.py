@@ -62,6 +62,14 @@ def lib():
         L.orc_greeks_pathwise.argtypes = [_d, _d, _d, _d, _d, _i, _u, _u, _u, _pd, _pd]
         L.orc_greeks_fd.argtypes = [_i, _pd, _d, _d, _u, _u, C.POINTER(C.c_int), _i, _u, _d,
                                     _pd]
+        L.orc_barrier_mc.argtypes = [_d] * 7 + [_i, _i, _i, _d, _i, _u, _u, _u]
+        L.orc_barrier_mc.restype = _d
+        L.orc_lookback_mc.argtypes = [_d] * 6 + [_i, _i, _i, _d, _i, _d, _u, _u, _u]
+        L.orc_lookback_mc.restype = _d
+        L.orc_asian_mc.argtypes = [_d] * 6 + [_u, _i, _i, _i, _u, _u]
+        L.orc_asian_mc.restype = _d
+        L.orc_asian_floating_seasoned.argtypes = [_d] * 5 + [_u, _i, _i, _d, _d, _u, _u]
+        L.orc_asian_floating_seasoned.restype = _d
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -251,6 +259,41 @@ def greeks_fd(model, spec, payoff, strike, n_paths, n_steps, greeks, seed=42, bu
                              n_steps, ids, len(greeks), seed, bump_size, _p(out))
     assert rc == 0
     return {g: float(out[i]) for i, g in enumerate(greeks)}
+
+
+_BARRIER = {"up-and-out": (1, 1), "up-and-in": (1, 0), "down-and-out": (0, 1),
+            "down-and-in": (0, 0)}
+
+
+def barrier_mc(s, k, r, sigma, t, barrier, barrier_type, option_type, q=0.0, n_paths=100_000,
+               n_steps=252, seed=42, rebate=0.0, rebate_at_hit=False):
+    up, out = _BARRIER[barrier_type]
+    return lib().orc_barrier_mc(s, k, r, q, sigma, t, barrier, up, out,
+                                int(option_type == "call"), rebate, int(rebate_at_hit), n_paths,
+                                n_steps, seed)
+
+
+def lookback_mc(s, r, sigma, t, n_steps, strike_type, option_type, k=0.0, q=0.0,
+                n_paths=100_000, seed=42, running_max=None, running_min=None):
+    return lib().orc_lookback_mc(s, k, r, q, sigma, t, int(strike_type == "floating"),
+                                 int(option_type == "call"), int(running_max is not None),
+                                 running_max or 0.0, int(running_min is not None),
+                                 running_min or 0.0, n_paths, n_steps, seed)
+
+
+def asian_mc(s, k, r, sigma, t, n_steps, average_type, strike_type, option_type, q=0.0,
+             n_paths=100_000, seed=42):
+    return lib().orc_asian_mc(s, k, r, q, sigma, t, n_steps, int(average_type == "geometric"),
+                              int(strike_type == "fixed"), int(option_type == "call"), n_paths,
+                              seed)
+
+
+def asian_floating_seasoned(s, r, sigma, t, n_steps, average_type, option_type, a_spent, t1,
+                            q=0.0, n_paths=100_000, seed=42):
+    return lib().orc_asian_floating_seasoned(s, r, q, sigma, t, n_steps,
+                                             int(average_type == "geometric"),
+                                             int(option_type == "call"), a_spent, t1, n_paths,
+                                             seed)
 
 
 def num_threads():

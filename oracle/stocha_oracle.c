@@ -867,3 +867,222 @@ int orc_greeks_fd(int model, const double *spec, double sign, double strike, uin
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Path-dependent options on GBM: barrier.rs:286-414, asian.rs:44-355,        */
+/* lookback.rs:158-210.  The loops follow the reference: a knocked-out       */
+/* barrier path stops drawing at its hit; sums run in path order.            */
+/* ------------------------------------------------------------------------ */
+/* bs.rs:3-25 (Abramowitz & Stegun 7.1.26) */
+static double orc_erf(double x) {
+    double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741, a4 = -1.453152027,
+           a5 = 1.061405429, p = 0.3275911;
+    double sign = x < 0.0 ? -1.0 : 1.0;
+    x = fabs(x);
+    double t = 1.0 / (1.0 + p * x);
+    double y = 1.0 - (((((a5 * t + a4) * t) + a3) * t + a2) * t + a1) * t * exp(-x * x);
+    return sign * y;
+}
+static double orc_norm_cdf(double x) { return 0.5 * (1.0 + orc_erf(x * 0.70710678118654752440)); }
+static double orc_max0(double v) { return v > 0.0 ? v : 0.0; }
+
+/* barrier.rs:286-372; up/out/call flags; rebate_at_hit as in BarrierParams. */
+double orc_barrier_mc(double s0, double k, double r, double q, double sigma, double t,
+                      double barrier, int up, int out, int call, double rebate, int rebate_at_hit,
+                      uint64_t n_paths, uint64_t n_steps, uint64_t seed) {
+    double dt = t / (double)n_steps;
+    double drift = (r - q - 0.5 * sigma * sigma) * dt;
+    double diffusion = sigma * sqrt(dt);
+    double discount = exp(-r * t);
+    uint64_t block = (n_steps + 1024) * 2;
+    int already_hit = up ? s0 >= barrier : s0 <= barrier;
+    double *vals = (double *)malloc(sizeof(double) * (n_paths ? n_paths : 1));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++) {
+        pcg_t g;
+        pcg_for_path(&g, seed, block, (uint64_t)i);
+        double s = s0;
+        int64_t hit = -1;
+        if (already_hit) {
+            hit = 0;
+        } else {
+            for (uint64_t step = 1; step <= n_steps; step++) {
+                double z = zig_normal(&g);
+                s *= exp(drift + diffusion * z);
+                if (up ? s >= barrier : s <= barrier) {
+                    hit = (int64_t)step;
+                    break;
+                }
+            }
+        }
+        int knocked = hit >= 0;
+        if (knocked && !out && (uint64_t)hit < n_steps) {
+            for (uint64_t j = (uint64_t)hit; j < n_steps; j++) {
+                double z = zig_normal(&g);
+                s *= exp(drift + diffusion * z);
+            }
+        }
+        double intrinsic = orc_max0(call ? s - k : k - s);
+        double option = out ? (knocked ? 0.0 : intrinsic) : (knocked ? intrinsic : 0.0);
+        double rebate_pv = 0.0; /* rebate_path_pv, barrier.rs:374-400 */
+        if (rebate != 0.0) {
+            if (out && rebate_at_hit) {
+                rebate_pv = knocked ? rebate * exp(-r * ((double)hit * dt)) : 0.0;
+            } else if (out) {
+                rebate_pv = knocked ? rebate * exp(-r * ((double)n_steps * dt)) : 0.0;
+            } else {
+                rebate_pv = knocked ? 0.0 : rebate * exp(-r * ((double)n_steps * dt));
+            }
+        }
+        vals[i] = option + rebate_pv / discount;
+    }
+    double total = 0.0;
+    for (uint64_t i = 0; i < n_paths; i++) total += vals[i];
+    free(vals);
+    return discount * total / (double)n_paths;
+}
+
+/* lookback.rs:158-210; has_max/has_min select the Option values. */
+double orc_lookback_mc(double s0, double k, double r, double q, double sigma, double t,
+                       int floating, int call, int has_max, double running_max, int has_min,
+                       double running_min, uint64_t n_paths, uint64_t n_steps, uint64_t seed) {
+    double dt = t / (double)n_steps;
+    double drift = (r - q - 0.5 * sigma * sigma) * dt;
+    double diffusion = sigma * sqrt(dt);
+    double discount = exp(-r * t);
+    uint64_t block = (n_steps + 1024) * 2;
+    double m0 = has_max ? running_max : s0, n0 = has_min ? running_min : s0;
+    m0 = (m0 != m0) ? s0 : (m0 > s0 ? m0 : s0); /* f64::max / min ignore a NaN */
+    n0 = (n0 != n0) ? s0 : (n0 < s0 ? n0 : s0);
+    double *vals = (double *)malloc(sizeof(double) * (n_paths ? n_paths : 1));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++) {
+        pcg_t g;
+        pcg_for_path(&g, seed, block, (uint64_t)i);
+        double s = s0, s_max = m0, s_min = n0;
+        for (uint64_t j = 0; j < n_steps; j++) {
+            double z = zig_normal(&g);
+            s *= exp(drift + diffusion * z);
+            if (s > s_max) s_max = s;
+            if (s < s_min) s_min = s;
+        }
+        double pay;
+        if (floating) pay = call ? s - s_min : s_max - s;
+        else pay = call ? orc_max0(s_max - k) : orc_max0(k - s_min);
+        vals[i] = pay;
+    }
+    double total = 0.0;
+    for (uint64_t i = 0; i < n_paths; i++) total += vals[i];
+    free(vals);
+    return discount * total / (double)n_paths;
+}
+
+/* asian.rs:78-137 without seasoning (Kemna-Vorst as the reference writes it) */
+static double orc_asian_geo_analytical(double s, double k, double r, double q, double sigma,
+                                       double t, int call) {
+    double sigma_g = sigma / sqrt(3.0);
+    double mu_g = 0.5 * (r - q - 0.5 * sigma * sigma) + 0.5 * 0.5 * sigma_g * sigma_g;
+    double d1 = (log(s / k) + (mu_g + 0.5 * sigma_g * sigma_g) * t) / (sigma_g * sqrt(t));
+    double d2 = d1 - sigma_g * sqrt(t);
+    double df = exp(-r * t);
+    double price = call ? s * exp((mu_g - r) * t) * orc_norm_cdf(d1) - k * df * orc_norm_cdf(d2)
+                        : k * df * orc_norm_cdf(-d2) - s * exp((mu_g - r) * t) * orc_norm_cdf(-d1);
+    return orc_max0(price);
+}
+
+/* asian.rs:143-290 for an unseasoned option (the seasoned forms reduce to it
+ * or to asian_mc_floating_seasoned, below). */
+double orc_asian_mc(double s0, double k, double r, double q, double sigma, double t,
+                    uint64_t n_steps, int geometric, int fixed, int call, uint64_t n_paths,
+                    uint64_t seed) {
+    double dt = t / (double)n_steps;
+    double drift = (r - q - 0.5 * sigma * sigma) * dt;
+    double diffusion = sigma * sqrt(dt);
+    double discount = exp(-r * t);
+    uint64_t block = (n_steps + 1024) * 2;
+    int use_cv = !geometric && fixed;
+    double geo_an = use_cv ? orc_asian_geo_analytical(s0, k, r, q, sigma, t, call) : 0.0;
+    double *a = (double *)malloc(sizeof(double) * (n_paths ? n_paths : 1));
+    double *gv = (double *)malloc(sizeof(double) * (n_paths ? n_paths : 1));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++) {
+        pcg_t g;
+        pcg_for_path(&g, seed, block, (uint64_t)i);
+        double s = s0, sum_arith = 0.0, sum_log = 0.0;
+        for (uint64_t j = 0; j < n_steps; j++) {
+            double z = zig_normal(&g);
+            s *= exp(drift + diffusion * z);
+            sum_arith += s;
+            sum_log += log(s);
+        }
+        double avg_arith = sum_arith / (double)n_steps;
+        double avg_geo = exp(sum_log / (double)n_steps);
+        double pa, pg;
+        if (fixed) {
+            pa = call ? orc_max0(avg_arith - k) : orc_max0(k - avg_arith);
+            pg = call ? orc_max0(avg_geo - k) : orc_max0(k - avg_geo);
+        } else {
+            pa = call ? orc_max0(s - avg_arith) : orc_max0(avg_arith - s);
+            pg = call ? orc_max0(s - avg_geo) : orc_max0(avg_geo - s);
+        }
+        a[i] = pa;
+        gv[i] = pg;
+    }
+    double res;
+    if (use_cv && n_paths > 1) {
+        double n = (double)n_paths, sa = 0.0, sg = 0.0;
+        for (uint64_t i = 0; i < n_paths; i++) sa += a[i];
+        for (uint64_t i = 0; i < n_paths; i++) sg += gv[i];
+        double mean_arith = sa / n, mean_geo = sg / n, cov = 0.0, var_geo = 0.0;
+        for (uint64_t i = 0; i < n_paths; i++) {
+            cov += (a[i] - mean_arith) * (gv[i] - mean_geo);
+            var_geo += (gv[i] - mean_geo) * (gv[i] - mean_geo);
+        }
+        double beta = var_geo > 1e-30 ? cov / var_geo : 0.0;
+        double geo_mc_price = discount * mean_geo;
+        res = orc_max0(discount * mean_arith - beta * (geo_mc_price - geo_an));
+    } else {
+        double sa = 0.0;
+        for (uint64_t i = 0; i < n_paths; i++) sa += a[i];
+        res = orc_max0(discount * (sa / (double)n_paths));
+    }
+    free(a);
+    free(gv);
+    return res;
+}
+
+/* asian.rs:293-355 */
+double orc_asian_floating_seasoned(double s0, double r, double q, double sigma, double t,
+                                   uint64_t n_steps, int geometric, int call, double a_spent,
+                                   double t1, uint64_t n_paths, uint64_t seed) {
+    double remaining = t - t1;
+    double nr = round((double)n_steps * remaining / t);
+    uint64_t n_rem = nr >= 1.0 ? (uint64_t)nr : 1;
+    double dt = remaining / (double)n_rem;
+    double drift = (r - q - 0.5 * sigma * sigma) * dt;
+    double diffusion = sigma * sqrt(dt);
+    double discount = exp(-r * remaining);
+    double w_spent = t1 / t, w_remain = remaining / t;
+    uint64_t block = (n_rem + 1024) * 2;
+    double *vals = (double *)malloc(sizeof(double) * (n_paths ? n_paths : 1));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++) {
+        pcg_t g;
+        pcg_for_path(&g, seed, block, (uint64_t)i);
+        double s = s0, sum_arith = 0.0, sum_log = 0.0;
+        for (uint64_t j = 0; j < n_rem; j++) {
+            double z = zig_normal(&g);
+            s *= exp(drift + diffusion * z);
+            sum_arith += s;
+            sum_log += log(s);
+        }
+        double avg_remain = geometric ? exp(sum_log / (double)n_rem) : sum_arith / (double)n_rem;
+        double avg_t = geometric ? exp(log(a_spent) * w_spent + log(avg_remain) * w_remain)
+                                 : w_spent * a_spent + w_remain * avg_remain;
+        vals[i] = call ? orc_max0(s - avg_t) : orc_max0(avg_t - s);
+    }
+    double total = 0.0;
+    for (uint64_t i = 0; i < n_paths; i++) total += vals[i];
+    free(vals);
+    return discount * total / (double)n_paths;
+}

@@ -97,6 +97,22 @@ int orc_greeks_fd(int model, const double *spec, double sign, double strike, uin
                   uint64_t steps, const int *greeks, int n_greeks, uint64_t seed,
                   double bump_size, double *values);
 
+/* ---- path-dependent options on GBM (Monte Carlo) ----
+ * barrier.rs:286-372, lookback.rs:158-210, asian.rs:143-290 (unseasoned) and
+ * :293-355 (seasoned floating strike).  Flags are 0/1. */
+double orc_barrier_mc(double s0, double k, double r, double q, double sigma, double t,
+                      double barrier, int up, int out, int call, double rebate, int rebate_at_hit,
+                      uint64_t n_paths, uint64_t n_steps, uint64_t seed);
+double orc_lookback_mc(double s0, double k, double r, double q, double sigma, double t,
+                       int floating, int call, int has_max, double running_max, int has_min,
+                       double running_min, uint64_t n_paths, uint64_t n_steps, uint64_t seed);
+double orc_asian_mc(double s0, double k, double r, double q, double sigma, double t,
+                    uint64_t n_steps, int geometric, int fixed, int call, uint64_t n_paths,
+                    uint64_t seed);
+double orc_asian_floating_seasoned(double s0, double r, double q, double sigma, double t,
+                                   uint64_t n_steps, int geometric, int call, double a_spent,
+                                   double t1, uint64_t n_paths, uint64_t seed);
+
 /* Number of OpenMP threads the oracle will use. */
 int orc_num_threads(void);
 

@@ -11,6 +11,9 @@ errors:
     lsmc_american_option   (__init__.py:630-671)
     greeks_pathwise        (__init__.py:725-769)
     greeks_fd              (__init__.py:674-722)
+    barrier_price          (__init__.py:983-1060)
+    asian_price            (__init__.py:1063-1127)
+    lookback_price         (__init__.py:1130-1190)
 
 They run on the GPU through the C ABI of libstocha_gpu.so (hand-written HIP
 kernels); there is no CPU fallback.  Forms beside the drop-in API:
@@ -21,7 +24,7 @@ kernels); there is no CPU fallback.  Forms beside the drop-in API:
                            (stocha_amd.autograd wraps it as a torch.autograd.Function)
 
 The reference's other functions (RNG, sobol, halton, copulas, Hull-White,
-SABR, Heston COS / calibration, SSVI, exotics, var_cvar) are not
+SABR, Heston COS / calibration, SSVI, var_cvar) are not
 part of this package: calling them raises NotImplementedError.
 """
 from __future__ import annotations
@@ -44,6 +47,9 @@ __all__ = [
     "lsmc_american_option",
     "greeks_pathwise",
     "greeks_fd",
+    "barrier_price",
+    "asian_price",
+    "lookback_price",
     "european_price",
     "lsmc_american_option_ex",
     "basket_price",
@@ -125,6 +131,46 @@ def greeks_fd(model: str, params: dict, payoff, strike: float, n_paths: int, n_s
     return _S.greeks_fd(model=model, params=params, payoff=payoff, strike=strike,
                         n_paths=n_paths, n_steps=n_steps, greeks=greeks, seed=seed,
                         bump_size=bump_size)
+
+
+def barrier_price(s: float, k: float, r: float, sigma: float, t: float, barrier: float,
+                  barrier_type: str = "up-and-out", option_type: str = "call", q: float = 0.0,
+                  n_paths: int = 100_000, n_steps: int = 252, seed: int = 42,
+                  method: str = "auto", n_monitoring: int | None = None, rebate: float = 0.0,
+                  rebate_at_hit: bool = False) -> float:
+    """Barrier option, analytical (Reiner-Rubinstein, BGK discrete-monitoring
+    shift, rebates) or Monte Carlo on the GPU (reference:
+    python/stocha/__init__.py:983-1060)."""
+    return _S.barrier_price(s=s, k=k, r=r, sigma=sigma, t=t, barrier=barrier,
+                            barrier_type=barrier_type, option_type=option_type, q=q,
+                            n_paths=n_paths, n_steps=n_steps, seed=seed, method=method,
+                            n_monitoring=n_monitoring, rebate=rebate, rebate_at_hit=rebate_at_hit)
+
+
+def asian_price(s: float, k: float, r: float, sigma: float, t: float, n_steps: int = 252,
+                average_type: str = "arithmetic", strike_type: str = "fixed",
+                option_type: str = "call", q: float = 0.0, n_paths: int = 100_000,
+                seed: int = 42, method: str = "auto", running_avg: float | None = None,
+                time_elapsed: float | None = None) -> float:
+    """Asian option: Kemna-Vorst for the geometric fixed strike, Monte Carlo on
+    the GPU otherwise (geometric control variate for the arithmetic fixed
+    strike) (reference: python/stocha/__init__.py:1063-1127)."""
+    return _S.asian_price(s=s, k=k, r=r, sigma=sigma, t=t, n_steps=n_steps,
+                          average_type=average_type, strike_type=strike_type,
+                          option_type=option_type, q=q, n_paths=n_paths, seed=seed, method=method,
+                          running_avg=running_avg, time_elapsed=time_elapsed)
+
+
+def lookback_price(s: float, r: float, sigma: float, t: float, n_steps: int = 252,
+                   strike_type: str = "floating", option_type: str = "call", k: float = 0.0,
+                   q: float = 0.0, n_paths: int = 100_000, seed: int = 42, method: str = "auto",
+                   running_max: float | None = None, running_min: float | None = None) -> float:
+    """Lookback option: Goldman-Sosin-Gatto / Conze-Viswanathan closed forms or
+    Monte Carlo on the GPU (reference: python/stocha/__init__.py:1130-1190)."""
+    return _S.lookback_price(s=s, r=r, sigma=sigma, t=t, n_steps=n_steps,
+                             strike_type=strike_type, option_type=option_type, k=k, q=q,
+                             n_paths=n_paths, seed=seed, method=method, running_max=running_max,
+                             running_min=running_min)
 
 
 # ------------------------------------------------------------------ extras --
@@ -211,8 +257,7 @@ def _not_here(name):
 
 for _name in ("RNG", "sobol", "halton", "var_cvar", "gaussian_copula", "student_t_copula",
               "hull_white", "sabr_implied_vol", "sabr_calibrate", "heston_price",
-              "heston_calibrate", "ssvi_calibrate", "ssvi_implied_vol", "ssvi_local_vol",
-              "barrier_price", "asian_price", "lookback_price"):
+              "heston_calibrate", "ssvi_calibrate", "ssvi_implied_vol", "ssvi_local_vol"):
     globals()[_name] = _not_here(_name)
 del _name
 

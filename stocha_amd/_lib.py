@@ -55,6 +55,12 @@ _SIGS = {
     "stocha_greeks_fd": ([_i, _pd, _u, _ps, _ps, _d, _u, _u, C.POINTER(C.c_char_p), _u, _u, _d,
                           _pd, _ps, _sz], C.c_int),
     "stocha_terminal_prices": ([_i, _pd, _u, _ps, _u, _u, _u, _vp, _i, _ps, _sz], C.c_int),
+    "stocha_barrier_price": ([_d] * 6 + [_ps, _ps, _d, _u, _u, _u, _ps, _i, C.c_uint32, _d, _i,
+                                          _pd, _ps, _sz], C.c_int),
+    "stocha_asian_price": ([_d] * 5 + [_u, _ps, _ps, _ps, _d, _u, _u, _ps, _i, _d, _i, _d, _pd,
+                                        _ps, _sz], C.c_int),
+    "stocha_lookback_price": ([_d] * 4 + [_u, _ps, _ps, _d, _d, _u, _u, _ps, _i, _d, _i, _d, _pd,
+                                           _ps, _sz], C.c_int),
     "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _pd, _pd, _pd,
                                _ps, _sz], C.c_int),
     "stocha_lsmc_ex": ([_d] * 5 + [_u, _u, _i, _u, _u, _i, _pd, _pd, _ps, _sz], C.c_int),

@@ -342,3 +342,70 @@ def greeks_fd(model, params, payoff, strike, n_paths, n_steps, greeks, seed=42, 
         with np.errstate(divide="ignore", invalid="ignore"):
             prices[k] = float(np.float64(discount * total) / np.float64(pv.size))
     return _greeks_from_prices(names, prices, bumps)
+
+
+# ---------------------------------------------- path-dependent options --
+def _str(x, name):
+    if not isinstance(x, str):
+        raise TypeError(f"argument '{name}': '{type(x).__name__}' object cannot be converted to "
+                        "'PyString'")
+    return x.encode()
+
+
+def _opt_f64(x, name):
+    return (0, 0.0) if x is None else (1, _f64(x, name))
+
+
+def _call_price(fn, *args):
+    lib, eb = L.load(), L.ErrBuf()
+    out = np.zeros(1, dtype=np.float64)
+    L.check(getattr(lib, fn)(*args, _pd(out), eb.buf, eb.n), eb)
+    return float(out[0])
+
+
+def barrier_price(s, k, r, sigma, t, barrier, barrier_type="up-and-out", option_type="call",
+                  q=0.0, n_paths=100_000, n_steps=252, seed=42, method="auto",
+                  n_monitoring=None, rebate=0.0, rebate_at_hit=False):
+    """src/lib.rs:1551-1638 (signature :1549)."""
+    vals = [_f64(v, n) for v, n in ((s, "s"), (k, "k"), (r, "r"), (sigma, "sigma"), (t, "t"),
+                                     (barrier, "barrier"))]
+    bt, ot = _str(barrier_type, "barrier_type"), _str(option_type, "option_type")
+    q = _f64(q, "q")
+    n_paths, n_steps, seed = _uint(n_paths, "n_paths"), _uint(n_steps, "n_steps"), \
+        _uint(seed, "seed")
+    m = _str(method, "method")
+    has_mon, mon = (0, 0) if n_monitoring is None else (1, _uint(n_monitoring, "n_monitoring",
+                                                                 bits=32))
+    rebate, rebate_at_hit = _f64(rebate, "rebate"), _bool(rebate_at_hit, "rebate_at_hit")
+    return _call_price("stocha_barrier_price", *vals, bt, ot, q, n_paths, n_steps, seed, m,
+                       has_mon, mon, rebate, int(rebate_at_hit))
+
+
+def asian_price(s, k, r, sigma, t, n_steps=252, average_type="arithmetic", strike_type="fixed",
+                option_type="call", q=0.0, n_paths=100_000, seed=42, method="auto",
+                running_avg=None, time_elapsed=None):
+    """src/lib.rs:1647-1745 (signature :1645)."""
+    vals = [_f64(v, n) for v, n in ((s, "s"), (k, "k"), (r, "r"), (sigma, "sigma"), (t, "t"))]
+    n_steps = _uint(n_steps, "n_steps")
+    at, st = _str(average_type, "average_type"), _str(strike_type, "strike_type")
+    ot, q = _str(option_type, "option_type"), _f64(q, "q")
+    n_paths, seed, m = _uint(n_paths, "n_paths"), _uint(seed, "seed"), _str(method, "method")
+    ha, a = _opt_f64(running_avg, "running_avg")
+    he, e = _opt_f64(time_elapsed, "time_elapsed")
+    return _call_price("stocha_asian_price", *vals, n_steps, at, st, ot, q, n_paths, seed, m,
+                       ha, a, he, e)
+
+
+def lookback_price(s, r, sigma, t, n_steps=252, strike_type="floating", option_type="call",
+                   k=0.0, q=0.0, n_paths=100_000, seed=42, method="auto", running_max=None,
+                   running_min=None):
+    """src/lib.rs:1761-1848 (signature :1759)."""
+    vals = [_f64(v, n) for v, n in ((s, "s"), (r, "r"), (sigma, "sigma"), (t, "t"))]
+    n_steps = _uint(n_steps, "n_steps")
+    st, ot = _str(strike_type, "strike_type"), _str(option_type, "option_type")
+    k, q = _f64(k, "k"), _f64(q, "q")
+    n_paths, seed, m = _uint(n_paths, "n_paths"), _uint(seed, "seed"), _str(method, "method")
+    hx, mx = _opt_f64(running_max, "running_max")
+    hn, mn = _opt_f64(running_min, "running_min")
+    return _call_price("stocha_lookback_price", *vals, n_steps, st, ot, k, q, n_paths, seed, m,
+                       hx, mx, hn, mn)

@@ -180,15 +180,6 @@ std::string rust_f64(double v) {
 
 namespace {
 
-int check_device(const Err& err) {
-  int n = 0;
-  hipError_t e = hipGetDeviceCount(&n);
-  if (e != hipSuccess || n == 0)
-    return err.set(STOCHA_EDEVICE, "no HIP device available (%s)",
-                   e == hipSuccess ? "0 devices" : hipGetErrorName(e));
-  return STOCHA_OK;
-}
-
 // Runs a path-storing launch into host or device memory.
 int store_paths(int model, const void* mc, uint64_t seed, uint64_t block, uint64_t n_rows,
                 uint64_t base_rows, uint64_t steps, double* out, int where, const Err& err) {

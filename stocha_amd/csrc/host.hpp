@@ -87,6 +87,16 @@ struct DevBuf {
 //           cooperative kernels; every kernel leaves them as it found them
 //           (the barrier's generation word only grows)
 //   host    pinned host memory for results copied back
+// A loud error when no device is visible (the library has no CPU path).
+inline int check_device(const Err& err) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return err.set(STOCHA_EDEVICE, "no HIP device available (%s)",
+                   e == hipSuccess ? "0 devices" : hipGetErrorName(e));
+  return STOCHA_OK;
+}
+
 struct Workspace {
   void* dev = nullptr;
   size_t dev_bytes = 0;
