@@ -17,6 +17,7 @@
  *   stocha_barrier_price         <- src/lib.rs:1551-1638 (barrier_price)
  *   stocha_asian_price           <- src/lib.rs:1647-1745 (asian_price)
  *   stocha_lookback_price        <- src/lib.rs:1761-1848 (lookback_price)
+ *   stocha_rng_new / _uniform / _normal <- src/lib.rs:75-145 (class RNG)
  *
  * Conventions (all entry points):
  *   - Plain C types only.  Integer arguments the reference takes as usize/u64
@@ -266,6 +267,23 @@ int stocha_batch_plan_run(stocha_batch_plan *plan, double *price, double *std_er
 /* Tiles and persistent grid size of a plan (for tuning reports). */
 int stocha_batch_plan_info(const stocha_batch_plan *plan, uint64_t *n_tiles, uint64_t *grid);
 void stocha_batch_plan_destroy(stocha_batch_plan *plan);
+
+/* ---- the RNG class (src/lib.rs:75-184): one sequential stream ------------
+ * A generator is its 128-bit LCG state and increment, each as two uint64_t
+ * (low word first), kept by the caller.  stocha_rng_new: Pcg64Dxsm::new for a
+ * 128-bit seed (pcg64.rs:23-30; host only).  stocha_rng_uniform: n values of
+ * next_f64 (RNG.uniform, lib.rs:108-112).  stocha_rng_normal: n ziggurat
+ * normals in the stream's order (RNG.standard_normal, lib.rs:114-119), or
+ * loc + scale z with scaled != 0 (RNG.normal, lib.rs:121-145; "scale must be
+ * positive").  Both write new_state, the state after the draws consumed
+ * (*consumed, optional, for normals), and fill out (host or device per
+ * `where`) from parallel kernels whose values are the sequential stream's. */
+int stocha_rng_new(uint64_t seed_lo, uint64_t seed_hi, uint64_t *state, uint64_t *inc);
+int stocha_rng_uniform(const uint64_t *state, const uint64_t *inc, uint64_t n, double *out,
+                       int where, uint64_t *new_state, char *err, size_t errlen);
+int stocha_rng_normal(const uint64_t *state, const uint64_t *inc, uint64_t n, double loc,
+                      double scale, int scaled, double *out, int where, uint64_t *new_state,
+                      uint64_t *consumed, char *err, size_t errlen);
 
 /* ---- random-number primitives (pcg64.rs:23-43, normal.rs:18-74) ---------
  * Draws of the block-split streams of paths path0 .. path0+n_paths-1, where

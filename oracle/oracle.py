@@ -70,6 +70,12 @@ def lib():
         L.orc_asian_mc.restype = _d
         L.orc_asian_floating_seasoned.argtypes = [_d] * 5 + [_u, _i, _i, _d, _d, _u, _u]
         L.orc_asian_floating_seasoned.restype = _d
+        L.orc_rng_new.argtypes = [_u, _u, _pu]
+        L.orc_rng_new.restype = None
+        L.orc_rng_uniform.argtypes = [_pu, _u, _pd]
+        L.orc_rng_uniform.restype = None
+        L.orc_rng_normal.argtypes = [_pu, _u, _d, _d, _i, _pd]
+        L.orc_rng_normal.restype = None
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -294,6 +300,28 @@ def asian_floating_seasoned(s, r, sigma, t, n_steps, average_type, option_type, 
                                              int(average_type == "geometric"),
                                              int(option_type == "call"), a_spent, t1, n_paths,
                                              seed)
+
+
+class Rng:
+    """The reference's RNG class as a sequential CPU stream (lib.rs:75-184)."""
+
+    def __init__(self, seed=0):
+        self.st = np.zeros(4, np.uint64)
+        lib().orc_rng_new(seed & (2**64 - 1), seed >> 64, _p(self.st, _pu))
+
+    def state(self):
+        s = [int(v) for v in self.st]
+        return s[0] | (s[1] << 64), s[2] | (s[3] << 64)
+
+    def uniform(self, n):
+        out = np.empty(n, np.float64)
+        lib().orc_rng_uniform(_p(self.st, _pu), n, _p(out))
+        return out
+
+    def normal(self, n, loc=0.0, scale=1.0, scaled=False):
+        out = np.empty(n, np.float64)
+        lib().orc_rng_normal(_p(self.st, _pu), n, loc, scale, int(scaled), _p(out))
+        return out
 
 
 def num_threads():

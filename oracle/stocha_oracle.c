@@ -1086,3 +1086,49 @@ double orc_asian_floating_seasoned(double s0, double r, double q, double sigma, 
     free(vals);
     return discount * total / (double)n_paths;
 }
+
+/* ------------------------------------------------------------------------ */
+/* The RNG class (src/lib.rs:75-184): one sequential stream.                  */
+/* st[4] = state lo, state hi, increment lo, increment hi (in/out).           */
+/* ------------------------------------------------------------------------ */
+void orc_rng_new(uint64_t seed_lo, uint64_t seed_hi, uint64_t st[4]) {
+    /* pcg64.rs:23-30 with a 128-bit seed */
+    u128 s = ((u128)seed_hi << 64) | seed_lo;
+    u128 stream = s + (((u128)0xda3e39cb94b95bdbull << 64) | 0x7ef55da5c0374f5aull);
+    u128 inc = (stream << 1) | 1;
+    u128 state = (s + inc) * (u128)PCG_CHEAP_MUL + inc;
+    st[0] = (uint64_t)state;
+    st[1] = (uint64_t)(state >> 64);
+    st[2] = (uint64_t)inc;
+    st[3] = (uint64_t)(inc >> 64);
+}
+
+static void orc_rng_load(pcg_t *g, const uint64_t st[4]) {
+    g->state = ((u128)st[1] << 64) | st[0];
+    g->inc = ((u128)st[3] << 64) | st[2];
+}
+
+static void orc_rng_store(const pcg_t *g, uint64_t st[4]) {
+    st[0] = (uint64_t)g->state;
+    st[1] = (uint64_t)(g->state >> 64);
+}
+
+/* lib.rs:108-112 */
+void orc_rng_uniform(uint64_t st[4], uint64_t n, double *out) {
+    pcg_t g;
+    orc_rng_load(&g, st);
+    for (uint64_t i = 0; i < n; i++) out[i] = pcg_next_f64(&g);
+    orc_rng_store(&g, st);
+}
+
+/* lib.rs:114-145: sample_into, or sample_scaled (normal.rs:70-79) */
+void orc_rng_normal(uint64_t st[4], uint64_t n, double loc, double scale, int scaled,
+                    double *out) {
+    pcg_t g;
+    orc_rng_load(&g, st);
+    for (uint64_t i = 0; i < n; i++) {
+        double z = zig_normal(&g);
+        out[i] = scaled ? loc + scale * z : z;
+    }
+    orc_rng_store(&g, st);
+}

@@ -113,6 +113,14 @@ double orc_asian_floating_seasoned(double s0, double r, double q, double sigma, 
                                    uint64_t n_steps, int geometric, int call, double a_spent,
                                    double t1, uint64_t n_paths, uint64_t seed);
 
+/* ---- the RNG class (lib.rs:75-184): one sequential stream ----
+ * st[4] = state lo, state hi, increment lo, increment hi; uniform / normal
+ * advance st in place. */
+void orc_rng_new(uint64_t seed_lo, uint64_t seed_hi, uint64_t st[4]);
+void orc_rng_uniform(uint64_t st[4], uint64_t n, double *out);
+void orc_rng_normal(uint64_t st[4], uint64_t n, double loc, double scale, int scaled,
+                    double *out);
+
 /* Number of OpenMP threads the oracle will use. */
 int orc_num_threads(void);
 

@@ -14,6 +14,8 @@ errors:
     barrier_price          (__init__.py:983-1060)
     asian_price            (__init__.py:1063-1127)
     lookback_price         (__init__.py:1130-1190)
+    RNG                    (the class: uniform / standard_normal / normal on one
+                            sequential stream, drawn on the GPU; JSON state)
 
 They run on the GPU through the C ABI of libstocha_gpu.so (hand-written HIP
 kernels); there is no CPU fallback.  Forms beside the drop-in API:
@@ -23,7 +25,7 @@ kernels); there is no CPU fallback.  Forms beside the drop-in API:
     basket_price           multi-asset basket call with pathwise adjoint Greeks
                            (stocha_amd.autograd wraps it as a torch.autograd.Function)
 
-The reference's other functions (RNG, sobol, halton, copulas, Hull-White,
+The reference's other functions (sobol, halton, copulas, Hull-White,
 SABR, Heston COS / calibration, SSVI, var_cvar) are not
 part of this package: calling them raises NotImplementedError.
 """
@@ -37,7 +39,7 @@ import numpy as np
 
 from . import _lib
 from . import _stocha as _S
-from ._stocha import __version__
+from ._stocha import RNG, __version__
 
 __all__ = [
     "gbm",
@@ -47,6 +49,7 @@ __all__ = [
     "lsmc_american_option",
     "greeks_pathwise",
     "greeks_fd",
+    "RNG",
     "barrier_price",
     "asian_price",
     "lookback_price",
@@ -255,7 +258,7 @@ def _not_here(name):
     return f
 
 
-for _name in ("RNG", "sobol", "halton", "var_cvar", "gaussian_copula", "student_t_copula",
+for _name in ("sobol", "halton", "var_cvar", "gaussian_copula", "student_t_copula",
               "hull_white", "sabr_implied_vol", "sabr_calibrate", "heston_price",
               "heston_calibrate", "ssvi_calibrate", "ssvi_implied_vol", "ssvi_local_vol"):
     globals()[_name] = _not_here(_name)

@@ -12,6 +12,7 @@ run before any device work.
 from __future__ import annotations
 
 import ctypes as C
+import json as _json
 import operator
 
 import numpy as np
@@ -409,3 +410,116 @@ def lookback_price(s, r, sigma, t, n_steps=252, strike_type="floating", option_t
     hn, mn = _opt_f64(running_min, "running_min")
     return _call_price("stocha_lookback_price", *vals, n_steps, st, ot, k, q, n_paths, seed, m,
                        hx, mx, hn, mn)
+
+
+# -------------------------------------------------------------- RNG class --
+_M64 = (1 << 64) - 1
+
+
+def _u128_field(obj, key):
+    v = obj[key]
+    if isinstance(v, bool) or not isinstance(v, int) or v < 0 or v >= 1 << 128:
+        raise ValueError(f"invalid value for '{key}': expected u128")
+    return v
+
+
+class RNG:
+    """src/lib.rs:75-184: a PCG64DXSM generator (one sequential stream) whose
+    bulk draws are made on the GPU.  The stream position is kept on the host
+    as the 128-bit LCG state; save_state / from_state use the reference's
+    JSON (pcg64.rs:57-77: {"version":2,"rng":{"state":..,"increment":..},
+    "seed":..}, or the legacy {"seed":..})."""
+
+    def __init__(self, seed=0, algorithm="pcg64dxsm"):
+        seed = _uint(seed, "seed", bits=128)
+        if not isinstance(algorithm, str):
+            raise TypeError(f"argument 'algorithm': '{type(algorithm).__name__}' object cannot "
+                            "be converted to 'PyString'")
+        if algorithm not in ("pcg64dxsm", "default"):
+            raise ValueError(f"Unknown algorithm: '{algorithm}'. Supported: 'pcg64dxsm'")
+        st, inc = (C.c_uint64 * 2)(), (C.c_uint64 * 2)()
+        L.load().stocha_rng_new(seed & _M64, seed >> 64, st, inc)
+        self._seed = seed
+        self._state = (st[0] | (st[1] << 64))
+        self._inc = (inc[0] | (inc[1] << 64))
+
+    @property
+    def seed(self):
+        return self._seed
+
+    def _args(self):
+        st = (C.c_uint64 * 2)(self._state & _M64, self._state >> 64)
+        inc = (C.c_uint64 * 2)(self._inc & _M64, self._inc >> 64)
+        return st, inc, (C.c_uint64 * 2)()
+
+    def uniform(self, size):
+        """lib.rs:108-112: size draws of next_f64."""
+        size = _uint(size, "size")
+        out = np.empty(size, dtype=np.float64)
+        st, inc, new = self._args()
+        lib, eb = L.load(), L.ErrBuf()
+        L.check(lib.stocha_rng_uniform(st, inc, size, out.ctypes.data, L.STOCHA_HOST, new,
+                                       eb.buf, eb.n), eb)
+        self._state = new[0] | (new[1] << 64)
+        return out
+
+    def _normals(self, size, loc, scale, scaled):
+        out = np.empty(size, dtype=np.float64)
+        st, inc, new = self._args()
+        lib, eb = L.load(), L.ErrBuf()
+        L.check(lib.stocha_rng_normal(st, inc, size, loc, scale, int(scaled), out.ctypes.data,
+                                      L.STOCHA_HOST, new, None, eb.buf, eb.n), eb)
+        self._state = new[0] | (new[1] << 64)
+        return out
+
+    def standard_normal(self, size):
+        """lib.rs:114-119: size ziggurat normals (NormalSampler::sample_into)."""
+        return self._normals(_uint(size, "size"), 0.0, 1.0, False)
+
+    def normal(self, size, loc=0.0, scale=1.0):
+        """lib.rs:121-145: loc + scale * z per value (sample_scaled)."""
+        size, loc, scale = _uint(size, "size"), _f64(loc, "loc"), _f64(scale, "scale")
+        if scale <= 0.0:
+            raise ValueError("scale must be positive")
+        return self._normals(size, loc, scale, True)
+
+    def save_state(self):
+        """lib.rs:147-157 -> pcg64.rs:57-65."""
+        return _json.dumps({"version": 2, "rng": {"state": self._state, "increment": self._inc},
+                            "seed": self._seed}, separators=(",", ":"))
+
+    @staticmethod
+    def from_state(json):
+        """lib.rs:159-178 -> pcg64.rs:67-77: the full state, else the legacy
+        seed-only form; anything else raises ValueError."""
+        if not isinstance(json, str):
+            raise TypeError(f"argument 'json': '{type(json).__name__}' object cannot be "
+                            "converted to 'PyString'")
+        try:
+            obj = _json.loads(json)
+        except ValueError as e:
+            raise ValueError(str(e)) from None
+        if not isinstance(obj, dict):
+            raise ValueError("invalid type: expected a JSON object")
+        try:
+            v = obj["version"]
+            if isinstance(v, bool) or not isinstance(v, int) or not 0 <= v < 256:
+                raise ValueError("invalid value for 'version': expected u8")
+            rng = obj["rng"]
+            if not isinstance(rng, dict):
+                raise ValueError("invalid type for 'rng'")
+            state, inc, seed = _u128_field(rng, "state"), _u128_field(rng, "increment"), \
+                _u128_field(obj, "seed")
+            r = RNG.__new__(RNG)
+            r._seed, r._state, r._inc = seed, state, inc
+            return r
+        except (KeyError, ValueError):
+            pass
+        try:
+            seed = _u128_field(obj, "seed")
+        except KeyError:
+            raise ValueError("missing field `seed`") from None
+        return RNG(seed)
+
+    def __repr__(self):
+        return f"RNG(seed={self._seed}, algorithm='pcg64dxsm')"
