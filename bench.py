@@ -254,6 +254,20 @@ def main():
                         "path_steps_per_s": ps / te}
         out["exotics_mc_1Mx252"] = ex
 
+        # RNG class bulk draws of one sequential stream, 100M values each
+        # (kernel time: the GPU work of the call; wall time adds the copy of
+        # 800 MB into pageable host memory)
+        rng_ex = {}
+        nbulk = 100_000_000
+        for name, fn in (("uniform", lambda: S.RNG(SEED).uniform(nbulk)),
+                         ("standard_normal", lambda: S.RNG(SEED).standard_normal(nbulk))):
+            tw = time_calls(fn, 2, 1)
+            kb = kernel_ms(S, fn, reps=2)
+            rng_ex[name] = {"values": nbulk, "wall_ms": tw * 1e3, "kernel_ms": kb,
+                            "values_per_s_kernel": nbulk / (kb * 1e-3),
+                            "hbm_gbs_written": nbulk * 8 / (kb * 1e-3) / 1e9}
+        out["rng_class_100M"] = rng_ex
+
         # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
         from stocha_amd import batch as B
         inst = B.config4_instances(4096, seed=SEED)
