@@ -18,6 +18,8 @@
  *   stocha_asian_price           <- src/lib.rs:1647-1745 (asian_price)
  *   stocha_lookback_price        <- src/lib.rs:1761-1848 (lookback_price)
  *   stocha_rng_new / _uniform / _normal <- src/lib.rs:75-145 (class RNG)
+ *   stocha_hull_white            <- src/lib.rs:606-658   (hull_white)
+ *   stocha_gaussian_copula       <- src/lib.rs:538-558   (gaussian_copula)
  *
  * Conventions (all entry points):
  *   - Plain C types only.  Integer arguments the reference takes as usize/u64
@@ -267,6 +269,22 @@ int stocha_batch_plan_run(stocha_batch_plan *plan, double *price, double *std_er
 /* Tiles and persistent grid size of a plan (for tuning reports). */
 int stocha_batch_plan_info(const stocha_batch_plan *plan, uint64_t *n_tiles, uint64_t *grid);
 void stocha_batch_plan_destroy(stocha_batch_plan *plan);
+
+/* src/lib.rs:606-658 -> hull_white.rs:26-52: short-rate paths, exact
+ * transition, one normal per step on block-split streams (block
+ * (steps + 1024) * 4).  out: (n_paths, steps+1) f64 row-major, column 0 = r0. */
+int stocha_hull_white(double r0, double a, double theta, double sigma, double t, uint64_t steps,
+                      uint64_t n_paths, uint64_t seed, double *out, int where, char *err,
+                      size_t errlen);
+
+/* src/lib.rs:538-558 -> copula/gaussian.rs:15-73: n_samples rows of
+ * Phi(L z), z the next dim normals of ONE stream Pcg64Dxsm::new(seed) (the
+ * RNG class's stream, decoded in parallel), L = cholesky(corr); corr is
+ * (rows, cols) row-major on the host, checked like the reference (square,
+ * unit diagonal, symmetric, positive definite).  out: (n_samples, dim) f64;
+ * out == NULL validates only. */
+int stocha_gaussian_copula(const double *corr, uint64_t rows, uint64_t cols, uint64_t n_samples,
+                           uint64_t seed, double *out, int where, char *err, size_t errlen);
 
 /* ---- the RNG class (src/lib.rs:75-184): one sequential stream ------------
  * A generator is its 128-bit LCG state and increment, each as two uint64_t

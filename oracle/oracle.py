@@ -76,6 +76,9 @@ def lib():
         L.orc_rng_uniform.restype = None
         L.orc_rng_normal.argtypes = [_pu, _u, _d, _d, _i, _pd]
         L.orc_rng_normal.restype = None
+        L.orc_hull_white.argtypes = [_d] * 5 + [_u, _u, _u, _pd]
+        L.orc_gaussian_copula.argtypes = [_pd, _u, _u, _u, _pd]
+        L.orc_gaussian_copula.restype = None
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -322,6 +325,22 @@ class Rng:
         out = np.empty(n, np.float64)
         lib().orc_rng_normal(_p(self.st, _pu), n, loc, scale, int(scaled), _p(out))
         return out
+
+
+def hull_white(r0, a, theta, sigma, t, steps, n_paths, seed=42):
+    out = np.empty((n_paths, steps + 1), np.float64)
+    assert lib().orc_hull_white(r0, a, theta, sigma, t, steps, n_paths, seed, _p(out)) == 0
+    return out
+
+
+def gaussian_copula(corr, n_samples, seed=42):
+    corr = np.ascontiguousarray(corr, np.float64)
+    d = corr.shape[0]
+    L = cholesky(corr) if d else np.zeros((0, 0))
+    L = np.ascontiguousarray(L, np.float64)
+    out = np.empty((n_samples, d), np.float64)
+    lib().orc_gaussian_copula(_p(L), d, n_samples, seed, _p(out))
+    return out
 
 
 def num_threads():

@@ -1132,3 +1132,58 @@ void orc_rng_normal(uint64_t st[4], uint64_t n, double loc, double scale, int sc
     }
     orc_rng_store(&g, st);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Hull-White one-factor, exact transition: hull_white.rs:26-52              */
+/* ------------------------------------------------------------------------ */
+int orc_hull_white(double r0, double a, double theta, double sigma, double t, uint64_t steps,
+                   uint64_t n_paths, uint64_t seed, double *out) {
+    double dt = t / (double)steps;
+    double e = exp(-a * dt);
+    double mean_reversion = (theta / a) * (1.0 - e);
+    double vol = sigma * sqrt((1.0 - e * e) / (2.0 * a));
+    uint64_t block = (steps + 1024) * 4;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n_paths; i++) {
+        pcg_t g;
+        pcg_for_path(&g, seed, block, (uint64_t)i);
+        double *row = out + (uint64_t)i * (steps + 1);
+        row[0] = r0;
+        for (uint64_t s = 0; s < steps; s++) {
+            double z = zig_normal(&g);
+            row[s + 1] = row[s] * e + mean_reversion + vol * z;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gaussian copula: copula/gaussian.rs:15-73 on one stream new(seed);         */
+/* L is the lower Cholesky factor (d x d row-major), computed by the caller  */
+/* with orc_cholesky.                                                        */
+/* ------------------------------------------------------------------------ */
+static double orc_norm_cdf_2617(double x) { /* dist/normal_cdf.rs:3-21 */
+    const double P = 0.2316419;
+    const double B[5] = {0.319381530, -0.356563782, 1.781477937, -1.821255978, 1.330274429};
+    double t = 1.0 / (1.0 + P * fabs(x));
+    double poly = t * (B[0] + t * (B[1] + t * (B[2] + t * (B[3] + t * B[4]))));
+    double pdf = exp(-0.5 * x * x) / sqrt(2.0 * 3.14159265358979323846);
+    double cdf_pos = 1.0 - pdf * poly;
+    return x >= 0.0 ? cdf_pos : 1.0 - cdf_pos;
+}
+
+void orc_gaussian_copula(const double *L, uint64_t d, uint64_t n_samples, uint64_t seed,
+                         double *out) {
+    pcg_t g;
+    pcg_new(&g, seed);
+    double *z = (double *)malloc(sizeof(double) * (d ? d : 1));
+    for (uint64_t i = 0; i < n_samples; i++) {
+        for (uint64_t k = 0; k < d; k++) z[k] = zig_normal(&g);
+        for (uint64_t j = 0; j < d; j++) {
+            double xj = 0.0;
+            for (uint64_t k = 0; k <= j; k++) xj += L[j * d + k] * z[k];
+            out[i * d + j] = orc_norm_cdf_2617(xj);
+        }
+    }
+    free(z);
+}

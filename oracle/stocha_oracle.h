@@ -121,6 +121,13 @@ void orc_rng_uniform(uint64_t st[4], uint64_t n, double *out);
 void orc_rng_normal(uint64_t st[4], uint64_t n, double loc, double scale, int scaled,
                     double *out);
 
+/* hull_white.rs:26-52; out (n_paths, steps+1). */
+int orc_hull_white(double r0, double a, double theta, double sigma, double t, uint64_t steps,
+                   uint64_t n_paths, uint64_t seed, double *out);
+/* copula/gaussian.rs:15-73 given the Cholesky factor L (d x d). */
+void orc_gaussian_copula(const double *L, uint64_t d, uint64_t n_samples, uint64_t seed,
+                         double *out);
+
 /* Number of OpenMP threads the oracle will use. */
 int orc_num_threads(void);
 

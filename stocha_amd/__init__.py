@@ -16,6 +16,8 @@ errors:
     lookback_price         (__init__.py:1130-1190)
     RNG                    (the class: uniform / standard_normal / normal on one
                             sequential stream, drawn on the GPU; JSON state)
+    hull_white             (__init__.py:451-492)
+    gaussian_copula        (__init__.py:400-423)
 
 They run on the GPU through the C ABI of libstocha_gpu.so (hand-written HIP
 kernels); there is no CPU fallback.  Forms beside the drop-in API:
@@ -25,8 +27,8 @@ kernels); there is no CPU fallback.  Forms beside the drop-in API:
     basket_price           multi-asset basket call with pathwise adjoint Greeks
                            (stocha_amd.autograd wraps it as a torch.autograd.Function)
 
-The reference's other functions (sobol, halton, copulas, Hull-White,
-SABR, Heston COS / calibration, SSVI, var_cvar) are not
+The reference's other functions (sobol, halton, the Student-t copula, SABR,
+Heston COS / calibration, SSVI, var_cvar) are not
 part of this package: calling them raises NotImplementedError.
 """
 from __future__ import annotations
@@ -50,6 +52,8 @@ __all__ = [
     "greeks_pathwise",
     "greeks_fd",
     "RNG",
+    "hull_white",
+    "gaussian_copula",
     "barrier_price",
     "asian_price",
     "lookback_price",
@@ -176,6 +180,20 @@ def lookback_price(s: float, r: float, sigma: float, t: float, n_steps: int = 25
                              running_min=running_min)
 
 
+def hull_white(r0: float, a: float, theta: float, sigma: float, t: float, steps: int,
+               n_paths: int, seed: int = 42) -> np.ndarray:
+    """Hull-White one-factor short-rate paths, exact transition; (n_paths,
+    steps + 1), column 0 = r0 (reference: python/stocha/__init__.py:451-492)."""
+    return _S.hull_white(r0=r0, a=a, theta=theta, sigma=sigma, t=t, steps=steps,
+                         n_paths=n_paths, seed=seed)
+
+
+def gaussian_copula(corr: np.ndarray, n_samples: int, seed: int = 42) -> np.ndarray:
+    """Gaussian copula samples Phi(L z) from one PCG64DXSM stream; (n_samples,
+    dim) in (0, 1) (reference: python/stocha/__init__.py:400-423)."""
+    return _S.gaussian_copula(corr=corr, n_samples=n_samples, seed=seed)
+
+
 # ------------------------------------------------------------------ extras --
 _MODELS = {"gbm": (_lib.MODEL_GBM, ("s0", "mu", "sigma")),
            "heston": (_lib.MODEL_HESTON, ("s0", "v0", "mu", "kappa", "theta", "xi", "rho")),
@@ -258,8 +276,7 @@ def _not_here(name):
     return f
 
 
-for _name in ("sobol", "halton", "var_cvar", "gaussian_copula", "student_t_copula",
-              "hull_white", "sabr_implied_vol", "sabr_calibrate", "heston_price",
+for _name in ("sobol", "halton", "var_cvar", "student_t_copula", "sabr_implied_vol", "sabr_calibrate", "heston_price",
               "heston_calibrate", "ssvi_calibrate", "ssvi_implied_vol", "ssvi_local_vol"):
     globals()[_name] = _not_here(_name)
 del _name

@@ -523,3 +523,35 @@ class RNG:
 
     def __repr__(self):
         return f"RNG(seed={self._seed}, algorithm='pcg64dxsm')"
+
+
+# ----------------------------------------------- Hull-White, Gaussian copula --
+def hull_white(r0, a, theta, sigma, t, steps, n_paths, seed=42):
+    """src/lib.rs:606-658 (signature :605).  Returns (n_paths, steps+1) float64."""
+    vals = [_f64(v, n) for v, n in ((r0, "r0"), (a, "a"), (theta, "theta"), (sigma, "sigma"),
+                                     (t, "t"))]
+    steps, n_paths, seed = _uint(steps, "steps"), _uint(n_paths, "n_paths"), _uint(seed, "seed")
+    lib, eb = L.load(), L.ErrBuf()
+    L.check(lib.stocha_hull_white(*vals, steps, n_paths, seed, None, L.STOCHA_HOST, eb.buf, eb.n),
+            eb)
+    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    L.check(lib.stocha_hull_white(*vals, steps, n_paths, seed, out.ctypes.data, L.STOCHA_HOST,
+                                  eb.buf, eb.n), eb)
+    return out
+
+
+def gaussian_copula(corr, n_samples, seed=42):
+    """src/lib.rs:538-558 (signature :537).  Returns (n_samples, dim) float64."""
+    if not isinstance(corr, np.ndarray) or corr.dtype != np.float64:
+        raise TypeError("argument 'corr': expected a numpy.ndarray of float64")
+    if corr.ndim != 2:
+        raise TypeError(f"argument 'corr': expected a 2-dimensional array, got {corr.ndim}")
+    n_samples, seed = _uint(n_samples, "n_samples"), _uint(seed, "seed")
+    c = np.ascontiguousarray(corr)
+    lib, eb = L.load(), L.ErrBuf()
+    L.check(lib.stocha_gaussian_copula(_pd(c), c.shape[0], c.shape[1], n_samples, seed, None,
+                                       L.STOCHA_HOST, eb.buf, eb.n), eb)
+    out = np.empty((n_samples, c.shape[0]), dtype=np.float64)
+    L.check(lib.stocha_gaussian_copula(_pd(c), c.shape[0], c.shape[1], n_samples, seed,
+                                       out.ctypes.data, L.STOCHA_HOST, eb.buf, eb.n), eb)
+    return out

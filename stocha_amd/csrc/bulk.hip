@@ -32,7 +32,12 @@
 // the host advances the generator by the draws consumed.
 #include <hipcub/hipcub.hpp>
 
+#include <math.h>
+
+#include <vector>
+
 #include "host.hpp"
+#include "multi.hpp"
 #include "zigsm.hpp"
 
 namespace stocha {
@@ -216,6 +221,122 @@ int place(double* out, int where, size_t bytes, double** dev, Workspace** ws, si
   return STOCHA_OK;
 }
 
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// n normals of the stream (state, inc) on the device: into ext (device) when
+// given, else at the start of the workspace; `front` bytes at the start of the
+// workspace are left to the caller (they hold the normals when ext is null).
+// *ws_out is the workspace the call used; *used the draws consumed.
+int decode_normals(const uint64_t state[2], const uint64_t inc[2], uint64_t n, double loc,
+                   double scale, int scaled, size_t front, double* ext, Workspace** ws_out,
+                   uint64_t* used, const Err& err) {
+  hipStream_t st = globals().stream;
+  StreamDesc sd;
+  seq_desc(state, inc, kChunk, &sd);
+  // positions to decode: ~1.0296 draws per normal, with margin; grown if short
+  uint64_t n_chunks = (n + n / 25 + 2 * kChunk) / kChunk + 1;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    if (n_chunks > 0x7fffffffull) return err.set(STOCHA_EINVAL, "size too large");
+    const unsigned grid = (unsigned)((n_chunks + kBulkBlock - 1) / kBulkBlock);
+    const size_t nc = n_chunks;
+    size_t tmp = 0;
+    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const uint64_t*)nullptr,
+                                                (uint64_t*)nullptr, (int)nc, st));
+    const size_t extra = al256(4 * nc) + 6 * al256(8 * nc) + 256 + al256(tmp);
+    Workspace* ws = nullptr;
+    if (int rc = workspace(al256(front) + extra, 0, &ws, err)) return rc;
+    double* dev = ext ? ext : static_cast<double*>(ws->dev);
+    char* w = static_cast<char*>(ws->dev) + al256(front);
+    uint32_t* spec_cnt = reinterpret_cast<uint32_t*>(w);
+    w += al256(4 * nc);
+    uint64_t* spec_mask = reinterpret_cast<uint64_t*>(w);
+    w += al256(8 * nc);
+    uint64_t* spec_exit = reinterpret_cast<uint64_t*>(w);
+    w += al256(8 * nc);
+    uint64_t* ex[2] = {reinterpret_cast<uint64_t*>(w),
+                       reinterpret_cast<uint64_t*>(w + al256(8 * nc))};
+    w += 2 * al256(8 * nc);
+    uint64_t* cnt = reinterpret_cast<uint64_t*>(w);
+    w += al256(8 * nc);
+    uint64_t* off = reinterpret_cast<uint64_t*>(w);
+    w += al256(8 * nc);
+    unsigned* flag = reinterpret_cast<unsigned*>(w);
+    uint64_t* cons = reinterpret_cast<uint64_t*>(w + 64);
+    w += 256;
+    void* scan_tmp = w;
+
+    KernelTimer timer(st);
+    timer.start();
+    norm_spec_kernel<<<grid, kBulkBlock, 0, st>>>(sd, n_chunks, spec_cnt, spec_mask, spec_exit);
+    STOCHA_HIP(hipGetLastError());
+    // resolve until no chunk's exit moves (the speculative exits first)
+    const uint64_t* prev = spec_exit;
+    int cur = 0;
+    uint64_t rounds = 0;
+    for (;;) {
+      STOCHA_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned), st));
+      norm_resolve_kernel<<<grid, kBulkBlock, 0, st>>>(sd, n_chunks, spec_cnt, spec_mask,
+                                                       spec_exit, prev, cnt, ex[cur], flag);
+      STOCHA_HIP(hipGetLastError());
+      unsigned h = 0;
+      STOCHA_HIP(hipMemcpyAsync(&h, flag, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+      STOCHA_HIP(hipStreamSynchronize(st));
+      if (!h) break;
+      if (++rounds > n_chunks)
+        return err.set(STOCHA_EDEVICE, "normal chunk resolution did not settle");
+      prev = ex[cur];
+      cur ^= 1;
+    }
+    // prev: the exits whose entries the final resolve used
+    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, off, (int)nc, st));
+    uint64_t last_off = 0, last_cnt = 0;
+    STOCHA_HIP(hipMemcpyAsync(&last_off, off + nc - 1, 8, hipMemcpyDeviceToHost, st));
+    STOCHA_HIP(hipMemcpyAsync(&last_cnt, cnt + nc - 1, 8, hipMemcpyDeviceToHost, st));
+    STOCHA_HIP(hipStreamSynchronize(st));
+    if (last_off + last_cnt < n) {  // too few positions: decode more
+      n_chunks += n_chunks / 4 + 4;
+      continue;
+    }
+    norm_emit_kernel<<<grid, kBulkBlock, 0, st>>>(sd, n_chunks, prev, cnt, off, n, loc, scale,
+                                                  scaled, dev, cons);
+    STOCHA_HIP(hipGetLastError());
+    timer.stop(3);
+    STOCHA_HIP(hipMemcpyAsync(used, cons, 8, hipMemcpyDeviceToHost, st));
+    STOCHA_HIP(hipStreamSynchronize(st));
+    *ws_out = ws;
+    return STOCHA_OK;
+  }
+  return err.set(STOCHA_EDEVICE, "could not decode enough normals");
+}
+
+// normal_cdf.rs:3-21 (Abramowitz & Stegun 26.2.17)
+__device__ __forceinline__ double norm_cdf_2617(double x) {
+  const double p = 0.2316419;
+  const double b0 = 0.319381530, b1 = -0.356563782, b2 = 1.781477937, b3 = -1.821255978,
+               b4 = 1.330274429;
+  const double t = 1.0 / (1.0 + p * fabs(x));
+  const double poly = t * (b0 + t * (b1 + t * (b2 + t * (b3 + t * b4))));
+  const double pdf = exp(-0.5 * x * x) / sqrt(2.0 * M_PI);
+  const double cdf_pos = 1.0 - pdf * poly;
+  return x >= 0.0 ? cdf_pos : 1.0 - cdf_pos;
+}
+
+// gaussian.rs:58-70: x_j = sum_{k<=j} L[j][k] z_k from 0.0 in k order
+__global__ __launch_bounds__(kBulkBlock) void gauss_copula_kernel(const double* __restrict__ z,
+                                                                  const double* __restrict__ L,
+                                                                  const uint64_t n,
+                                                                  const uint32_t d,
+                                                                  double* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (i >= n) return;
+  const double* zi = z + i * d;
+  for (uint32_t j = 0; j < d; ++j) {
+    double xj = 0.0;
+    for (uint32_t k = 0; k <= j; ++k) xj += L[(uint64_t)j * d + k] * zi[k];
+    out[i * d + j] = norm_cdf_2617(xj);
+  }
+}
+
 }  // namespace
 }  // namespace stocha
 
@@ -282,91 +403,75 @@ int stocha_rng_normal(const uint64_t* state, const uint64_t* inc, uint64_t n, do
   std::lock_guard<std::mutex> lk(globals().mu);
   if (int rc = check_device(err)) return rc;
   hipStream_t st = globals().stream;
-  StreamDesc sd;
-  seq_desc(state, inc, kChunk, &sd);
-  // positions to decode: ~1.0296 draws per normal, with margin; grown if short
-  uint64_t n_chunks = (n + n / 25 + 2 * kChunk) / kChunk + 1;
-  for (int attempt = 0; attempt < 8; ++attempt) {
-    if (n_chunks > (1ull << 40)) return err.set(STOCHA_EINVAL, "size too large");
-    const uint64_t grid64 = (n_chunks + kBulkBlock - 1) / kBulkBlock;
-    if (grid64 > 0x7fffffffull) return err.set(STOCHA_EINVAL, "size too large for one launch");
-    const unsigned grid = (unsigned)grid64;
-    // workspace: [out (host placement)] spec_cnt u32, spec_mask, spec_exit,
-    // exits x2, cnt, off (u64), flag, consumed, scan temp
-    const size_t nc = n_chunks;
-    size_t tmp = 0;
-    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const uint64_t*)nullptr,
-                                                (uint64_t*)nullptr, (int)nc, st));
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t extra = al(4 * nc) + 6 * al(8 * nc) + 256 + al(tmp);
-    Workspace* ws = nullptr;
-    double* dev = nullptr;
-    if (nc > 0x7fffffffull) return err.set(STOCHA_EINVAL, "size too large for one scan");
-    if (int rc = place(out, where, sizeof(double) * n, &dev, &ws, extra, err)) return rc;
-    char* w = static_cast<char*>(ws->dev) + (where == STOCHA_DEVICE ? 0 : al(sizeof(double) * n));
-    uint32_t* spec_cnt = reinterpret_cast<uint32_t*>(w);
-    w += al(4 * nc);
-    uint64_t* spec_mask = reinterpret_cast<uint64_t*>(w);
-    w += al(8 * nc);
-    uint64_t* spec_exit = reinterpret_cast<uint64_t*>(w);
-    w += al(8 * nc);
-    uint64_t* ex[2] = {reinterpret_cast<uint64_t*>(w), reinterpret_cast<uint64_t*>(w + al(8 * nc))};
-    w += 2 * al(8 * nc);
-    uint64_t* cnt = reinterpret_cast<uint64_t*>(w);
-    w += al(8 * nc);
-    uint64_t* off = reinterpret_cast<uint64_t*>(w);
-    w += al(8 * nc);
-    unsigned* flag = reinterpret_cast<unsigned*>(w);
-    uint64_t* cons = reinterpret_cast<uint64_t*>(w + 64);
-    w += 256;
-    void* scan_tmp = w;
+  const bool host = where != STOCHA_DEVICE;
+  Workspace* ws = nullptr;
+  uint64_t used = 0;
+  if (int rc = decode_normals(state, inc, n, loc, scale, scaled, host ? al256(8 * n) : 0,
+                              host ? nullptr : out, &ws, &used, err))
+    return rc;
+  if (host)
+    STOCHA_HIP(hipMemcpyAsync(out, ws->dev, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  STOCHA_HIP(hipStreamSynchronize(st));
+  advance_state(state, inc, used, new_state);
+  if (consumed) *consumed = used;
+  return STOCHA_OK;
+}
 
-    KernelTimer timer(st);
-    timer.start();
-    norm_spec_kernel<<<grid, kBulkBlock, 0, st>>>(sd, n_chunks, spec_cnt, spec_mask, spec_exit);
-    STOCHA_HIP(hipGetLastError());
-    // resolve until no chunk's exit moves (prev = the speculative exits first)
-    const uint64_t* prev = spec_exit;
-    int cur = 0;
-    uint64_t rounds = 0;
-    for (;;) {
-      STOCHA_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned), st));
-      norm_resolve_kernel<<<grid, kBulkBlock, 0, st>>>(sd, n_chunks, spec_cnt, spec_mask,
-                                                       spec_exit, prev, cnt, ex[cur], flag);
-      STOCHA_HIP(hipGetLastError());
-      unsigned h = 0;
-      STOCHA_HIP(hipMemcpyAsync(&h, flag, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-      STOCHA_HIP(hipStreamSynchronize(st));
-      if (!h) break;
-      if (++rounds > n_chunks) return err.set(STOCHA_EDEVICE, "normal chunk resolution did not settle");
-      prev = ex[cur];
-      cur ^= 1;
+// copula/gaussian.rs:15-73 (gaussian_copula_samples, binding lib.rs:538-558):
+// row i = Phi(L z_i) with z_i the normals i*d .. i*d+d-1 of the stream
+// Pcg64Dxsm::new(seed); corr is (rows, cols) row-major on the host.
+int stocha_gaussian_copula(const double* corr, uint64_t rows, uint64_t cols, uint64_t n_samples,
+                           uint64_t seed, double* out, int where, char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  const uint64_t d = rows;
+  if (cols != d) return err.set(STOCHA_EINVAL, "correlation matrix must be square");
+  for (uint64_t i = 0; i < d; ++i) {
+    const double v = corr[i * d + i];
+    if (fabs(v - 1.0) > 1e-10)
+      return err.set(STOCHA_EINVAL,
+                     "correlation matrix diagonal element [%llu][%llu] must be 1.0, got %s",
+                     (unsigned long long)i, (unsigned long long)i, rust_f64(v).c_str());
+    for (uint64_t j = i + 1; j < d; ++j) {
+      const double a = corr[i * d + j], b = corr[j * d + i];
+      if (fabs(a - b) > 1e-10)
+        return err.set(STOCHA_EINVAL,
+                       "correlation matrix is not symmetric: [%llu,%llu]=%s vs [%llu,%llu]=%s",
+                       (unsigned long long)i, (unsigned long long)j, rust_f64(a).c_str(),
+                       (unsigned long long)j, (unsigned long long)i, rust_f64(b).c_str());
     }
-    // prev holds the exits that define the entries used by the final resolve
-    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, off, (int)nc, st));
-    // total = off[last] + cnt[last]
-    uint64_t last_off = 0, last_cnt = 0;
-    STOCHA_HIP(hipMemcpyAsync(&last_off, off + nc - 1, 8, hipMemcpyDeviceToHost, st));
-    STOCHA_HIP(hipMemcpyAsync(&last_cnt, cnt + nc - 1, 8, hipMemcpyDeviceToHost, st));
-    STOCHA_HIP(hipStreamSynchronize(st));
-    if (last_off + last_cnt < n) {  // too few positions: decode more
-      n_chunks += n_chunks / 4 + 4;
-      continue;
-    }
-    norm_emit_kernel<<<grid, kBulkBlock, 0, st>>>(sd, n_chunks, prev, cnt, off, n, loc, scale,
-                                                  scaled, dev, cons);
-    STOCHA_HIP(hipGetLastError());
-    timer.stop(3);
-    uint64_t used = 0;
-    STOCHA_HIP(hipMemcpyAsync(&used, cons, 8, hipMemcpyDeviceToHost, st));
-    if (where != STOCHA_DEVICE)
-      STOCHA_HIP(hipMemcpyAsync(out, dev, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    STOCHA_HIP(hipStreamSynchronize(st));
-    advance_state(state, inc, used, new_state);
-    if (consumed) *consumed = used;
-    return STOCHA_OK;
   }
-  return err.set(STOCHA_EDEVICE, "could not decode enough normals");
+  std::vector<double> L(d * d, 0.0);
+  if (d > 0)
+    if (int rc = cholesky_host(corr, d, L.data(), err)) return rc;
+  if (!out) return STOCHA_OK;  // validate only
+  const uint64_t n = n_samples * d;
+  if (n == 0) return STOCHA_OK;
+  if (d > 0xffffffffull) return err.set(STOCHA_EINVAL, "dimension too large");
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  hipStream_t st = globals().stream;
+  uint64_t state[2], inc[2], used = 0;
+  stocha_rng_new(seed, 0, state, inc);
+  // front of the workspace: normals, L, and the output when it goes to the host
+  const bool host = where != STOCHA_DEVICE;
+  const size_t z_bytes = al256(8 * n), l_bytes = al256(8 * d * d);
+  Workspace* ws = nullptr;
+  if (int rc = decode_normals(state, inc, n, 0.0, 1.0, 0, z_bytes + l_bytes + (host ? al256(8 * n) : 0),
+                              nullptr, &ws, &used, err))
+    return rc;
+  char* base = static_cast<char*>(ws->dev);
+  const double* z = reinterpret_cast<const double*>(base);
+  double* dl = reinterpret_cast<double*>(base + z_bytes);
+  double* dout = host ? reinterpret_cast<double*>(base + z_bytes + l_bytes) : out;
+  STOCHA_HIP(hipMemcpyAsync(dl, L.data(), sizeof(double) * d * d, hipMemcpyHostToDevice, st));
+  const uint64_t blocks = (n_samples + kBulkBlock - 1) / kBulkBlock;
+  if (blocks > 0x7fffffffull) return err.set(STOCHA_EINVAL, "n_samples too large for one launch");
+  gauss_copula_kernel<<<(unsigned)blocks, kBulkBlock, 0, st>>>(z, dl, n_samples, (uint32_t)d,
+                                                               dout);
+  STOCHA_HIP(hipGetLastError());
+  if (host) STOCHA_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  STOCHA_HIP(hipStreamSynchronize(st));
+  return STOCHA_OK;
 }
 
 }  // extern "C"

@@ -523,6 +523,29 @@ int stocha_greeks_pathwise(double s0, double r, double sigma, double t, double s
   return STOCHA_OK;
 }
 
+// src/lib.rs:606-658 -> hull_white.rs:26-52
+int stocha_hull_white(double r0, double a, double theta, double sigma, double t, uint64_t steps,
+                      uint64_t n_paths, uint64_t seed, double* out, int where, char* errbuf,
+                      size_t errlen) {
+  Err err{errbuf, errlen};
+  if (a <= 0.0) return err.set(STOCHA_EINVAL, "a must be positive");
+  if (sigma <= 0.0) return err.set(STOCHA_EINVAL, "sigma must be positive");
+  if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
+  if (steps == 0 || n_paths == 0)
+    return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  if (!out) return STOCHA_OK;  // validate only
+  std::lock_guard<std::mutex> lk(globals().mu);
+  HullWhiteConst c;
+  const double dt = t / (double)steps;                     // hull_white.rs:27
+  const double e = exp(-a * dt);                           // :28
+  c.s0 = r0;
+  c.e = e;
+  c.mean_reversion = (theta / a) * (1.0 - e);              // :29
+  c.vol = sigma * sqrt((1.0 - e * e) / (2.0 * a));         // :30
+  return store_paths(kHullWhite, &c, seed, block_size(kHullWhite, steps), n_paths, n_paths, steps,
+                     out, where, err);
+}
+
 // src/lib.rs:1064-1117 -> greeks.rs:227-358 (greeks_fd, built-in payoff)
 int stocha_greeks_fd(int model, const double* params, uint64_t n_params, const char* scheme,
                      const char* payoff, double strike, uint64_t n_paths, uint64_t n_steps,

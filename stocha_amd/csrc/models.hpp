@@ -25,7 +25,7 @@
 
 namespace stocha {
 
-enum ModelKind : int { kGbm = 0, kHeston = 1, kMerton = 2, kHestonQe = 3 };
+enum ModelKind : int { kGbm = 0, kHeston = 1, kMerton = 2, kHestonQe = 3, kHullWhite = 4 };
 
 struct GbmConst {
   double s0, drift, diffusion;
@@ -59,11 +59,17 @@ struct HestonQeConst {
   double k0s, k1s, k2s, k3s, k4s;  // Andersen's log-price coefficients
 };
 
+// Hull-White one-factor, exact transition (hull_white.rs:26-52); s0 is r0.
+struct HullWhiteConst {
+  double s0, e, mean_reversion, vol;
+};
+
 // Stream budget per path (reference block sizes; QE keeps Euler's,
-// heston.rs:97).
+// heston.rs:97; Hull-White hull_white.rs:38).
 __host__ __device__ inline uint64_t block_size(int model, uint64_t steps) {
   return model == kGbm                            ? (steps + 1024) * 2
          : model == kHeston || model == kHestonQe ? (steps + 1024) * 6
+         : model == kHullWhite                    ? (steps + 1024) * 4
                                                   : (steps + 1024) * 8;
 }
 
@@ -125,6 +131,16 @@ struct HestonQe64 {
     s = exp(log_s);
     v = v_next;
   }
+  __device__ __forceinline__ double terminal() const { return s; }
+};
+
+// hull_white.rs:47-49: r(t+dt) = r(t) e + mean_reversion + vol z
+struct HullWhite64 {
+  static constexpr bool kFp32 = false;
+  HullWhiteConst c;
+  double s;  // the short rate
+  __device__ __forceinline__ HullWhite64(const HullWhiteConst& k, bool) : c(k), s(k.s0) {}
+  __device__ __forceinline__ void apply(double z) { s = s * c.e + c.mean_reversion + c.vol * z; }
   __device__ __forceinline__ double terminal() const { return s; }
 };
 

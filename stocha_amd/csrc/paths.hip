@@ -99,6 +99,10 @@ int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n
       paths_kernel<MertonSrc<false, Merton64>, Merton64, MertonConst, 4><<<grid, block, 0, st>>>(
           *(const MertonConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
+    case kHullWhite:
+      paths_kernel<GbmSrc<false>, HullWhite64, HullWhiteConst, 8><<<grid, block, 0, st>>>(
+          *(const HullWhiteConst*)mconst, sd, n_rows, base_rows, st32, d_out);
+      break;
     case kHestonQe:
       paths_kernel<HestonQeSrc<HestonQe64>, HestonQe64, HestonQeConst, 4>
           <<<grid, block, 0, st>>>(*(const HestonQeConst*)mconst, sd, n_rows, base_rows, st32,

@@ -189,7 +189,7 @@ def test_greeks_pathwise_validation(bad, greeks, msg):
 
 
 def test_out_of_scope_functions_say_so():
-    for name in ("sobol", "hull_white", "sabr_implied_vol"):
+    for name in ("sobol", "student_t_copula", "sabr_implied_vol"):
         with pytest.raises(NotImplementedError, match="hot path"):
             getattr(S, name)()
 
