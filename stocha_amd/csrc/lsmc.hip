@@ -116,6 +116,15 @@ __device__ __forceinline__ double spot_at(const void* paths, uint64_t n, uint64_
     return ((const double*)paths)[(date - 1) * n + i];
 }
 
+// The stored value itself (float for FP32 paths), for values kept in registers.
+template <bool kFp32>
+__device__ __forceinline__ typename std::conditional<kFp32, float, double>::type spot_raw(
+    const void* paths, uint64_t n, uint64_t date, uint32_t i) {
+  using T = typename std::conditional<kFp32, float, double>::type;
+  const T* col = (const T*)paths + (date - 1) * n;  // wave-uniform column base
+  return col[i];
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -444,18 +453,18 @@ __constant__ const double kBinom[kMaxMom][kMaxMom] = {
     {1, 6, 15, 20, 15, 6, 1}, {1, 7, 21, 35, 35, 21, 7, 1}, {1, 8, 28, 56, 70, 56, 28, 8, 1}};
 
 // Moments of x = (S - mean) / sdv = al u + be from the raw moments of
-// u = (S - c) / sc (tot[q] = sum u^q, q < nm; tot[nm + q] = sum cf u^q,
+// u = (S - c) / sc (tot[q] = sum u^q, q < nm; tot[yoff + q] = sum cf u^q,
 // q < ncols): sum x^e = sum_q C(e, q) al^q be^(e-q) tot[q], and the same for
-// sum cf x^e with tot[nm + q].  Lane e < nm of the calling wave writes g[e],
+// sum cf x^e with tot[yoff + q].  Lane e < nm of the calling wave writes g[e],
 // lane 16 + e (e < ncols) writes h[e].
-__device__ __forceinline__ void moments_x(const double* tot, int ncols, double al, double be,
-                                          double* g, double* h) {
+__device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff, double al,
+                                          double be, double* g, double* h) {
   const int nm = 2 * (ncols - 1) + 1;
   const int lane = threadIdx.x & 63;
   const bool is_g = lane < nm, is_h = lane >= 16 && lane < 16 + ncols;
   if (!is_g && !is_h) return;
   const int e = is_g ? lane : lane - 16;
-  const double* src = is_g ? tot : tot + nm;
+  const double* src = is_g ? tot : tot + yoff;
   double ap[kMaxMom], bp[kMaxMom];
   ap[0] = bp[0] = 1.0;
 #pragma unroll
@@ -635,8 +644,10 @@ __device__ __forceinline__ double exercise(double cf, double s, double k, int is
     const double x = (s - mean) * inv_sdv;
     double cont = 0.0;
 #pragma unroll
-    for (int j = 0; j < kMaxCols; ++j)
-      if (j < ncols) cont += c[j] * powi_rt(x, j);
+    for (int j = 0; j < kMaxCols; ++j) {
+      const double t = c[j] * powi_rt(x, j);
+      cont += j < ncols ? t : 0.0;  // a select: adding +0.0 changes no decision
+    }
     if (exv > (cont > 0.0 ? cont : 0.0)) return exv;
   }
   return cf;
@@ -651,39 +662,53 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   __shared__ double g_s[kMaxMom], h_s[kMaxCols];
   __shared__ double ms_s[2];
   __shared__ int skip_s;
-  const uint64_t G = (uint64_t)gridDim.x * kCoopBlock;
-  const uint64_t t0 = (uint64_t)blockIdx.x * kCoopBlock + threadIdx.x;
-  const int ncols = a.ncols, nm = 2 * (ncols - 1) + 1, na = nm + ncols;
+  // n <= kCoopMaxPpt x the grid's threads (< 2^32): path indices are 32-bit,
+  // and a spot's address is a per-date column base plus a 32-bit offset.
+  const uint32_t G = gridDim.x * kCoopBlock;
+  const uint32_t t0 = blockIdx.x * kCoopBlock + threadIdx.x;
+  // Accumulators in a fixed layout: [0, kMaxMom) sum u^e and
+  // [kMaxMom, kMaxAccB) sum cf u^e, all of them for every degree (static
+  // register indices, no branches on the degree); the solve reads the first
+  // 2 deg + 1 and deg + 1 of them.
+  const int ncols = a.ncols, na = kMaxAccB;
   const uint64_t pstride = (uint64_t)gridDim.x * na;
-  // Cash flows, and up to 8 paths per thread the spots of date+1, live in
-  // LDS (kPpt x 1024 doubles each, <= 128 KB with one block per CU); at 16
-  // paths per thread the spots of date+1 are read again (the previous date's
-  // column, still in the caches).  Registers stay free for the regression.
-  constexpr bool kKeepSpot = kPpt <= 8;
-  constexpr int kUnrollQ = kPpt <= 8 ? kPpt : 4;  // per-path loops: full unroll up to 8
+  // Cash flows live in LDS (kPpt x 1024 doubles, <= 128 KB with one block
+  // per CU).  The spots of date+1: FP32 paths above 4 per thread keep them
+  // in registers as float (kRegSpot, with the next date's column loaded a
+  // pass ahead); otherwise up to 8 per thread in LDS (kKeepSpot), and at 16
+  // FP64 paths per thread they are read again (the previous date's column,
+  // still in the caches).
+  constexpr bool kRegSpot = kFp32 && kPpt > 4;
+  constexpr bool kKeepSpot = !kRegSpot && kPpt <= 8;
+  constexpr int kUnrollQ = (kPpt <= 8 || kRegSpot) ? kPpt : 4;  // per-path loops
+  using SpotT = typename std::conditional<kFp32, float, double>::type;
   __shared__ double cf[kPpt][kCoopBlock];
   __shared__ double sp[kKeepSpot ? kPpt : 1][kCoopBlock];
+  SpotT cur[kRegSpot ? kPpt : 1];
   const int tx = threadIdx.x;
 #pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
-    const uint64_t i = t0 + q * G;
-    const double s0 = i < a.n ? spot_at<kFp32>(a.paths, a.n, a.steps, i) : 0.0;
+    const uint32_t i = t0 + q * G;
+    const SpotT r0 = i < a.n ? spot_raw<kFp32>(a.paths, a.n, a.steps, i) : SpotT(0);
+    const double s0 = (double)r0;
     if constexpr (kKeepSpot) sp[q][tx] = s0;
+    if constexpr (kRegSpot) cur[q] = r0;
     cf[q][tx] = i < a.n ? intrinsic(s0, a.k, a.is_put) : 0.0;  // lsmc.rs:45-47
   }
   bool have_ex = false;  // a regression was solved at date+1
   double ex_mean = 0.0, ex_std = 1.0, ex_inv = 1.0, ex_c[kMaxCols];
   for (int j = 0; j < kMaxCols; ++j) ex_c[j] = 0.0;
   double c = a.k, sc = a.k;  // moment variable u = (S - c) / sc
-  // Up to 4 paths per thread the spots of the next date are loaded at the end
-  // of a pass, so their latency overlaps the reduction, barrier and solve.
-  constexpr bool kPrefetch = kPpt <= 4;
-  double nxt[kPrefetch ? kPpt : 1];
+  // Up to 4 paths per thread, and with kRegSpot, the spots of the next date
+  // are loaded at the end of a pass, so their latency overlaps the
+  // reduction, barrier and solve.
+  constexpr bool kPrefetch = kPpt <= 4 || kRegSpot;
+  SpotT nxt[kPrefetch ? kPpt : 1];
   if constexpr (kPrefetch) {
 #pragma unroll
     for (int q = 0; q < kPpt; ++q) {
-      const uint64_t i = t0 + q * G;
-      nxt[q] = i < a.n ? spot_at<kFp32>(a.paths, a.n, a.steps - 1, i) : 0.0;
+      const uint32_t i = t0 + q * G;
+      nxt[q] = i < a.n ? spot_raw<kFp32>(a.paths, a.n, a.steps - 1, i) : SpotT(0);
     }
   }
   for (uint32_t date = a.steps - 1; date >= 1; --date) {
@@ -693,28 +718,29 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     const double inv_sc = 1.0 / sc;
 #pragma unroll kUnrollQ
     for (int q = 0; q < kPpt; ++q) {
-      const uint64_t i = t0 + q * G;
+      const uint32_t i = t0 + q * G;
       if (i < a.n) {
         double v = cf[q][tx];
         if (have_ex) {
-          const double s1 =
-              kKeepSpot ? sp[kKeepSpot ? q : 0][tx] : spot_at<kFp32>(a.paths, a.n, date + 1, i);
+          const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
+                            : kKeepSpot ? sp[kKeepSpot ? q : 0][tx]
+                                        : spot_at<kFp32>(a.paths, a.n, date + 1, i);
           v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
         }
         const double y = v * a.discount;  // lsmc.rs:52-54
         cf[q][tx] = y;
-        const double s = kPrefetch ? nxt[kPrefetch ? q : 0] : spot_at<kFp32>(a.paths, a.n, date, i);
+        const double s =
+            kPrefetch ? (double)nxt[kPrefetch ? q : 0] : spot_at<kFp32>(a.paths, a.n, date, i);
         if constexpr (kKeepSpot) sp[q][tx] = s;
+        if constexpr (kRegSpot) cur[q] = nxt[q];
         if (intrinsic(s, a.k, a.is_put) > 0.0) {
           const double u = (s - c) * inv_sc;
           double up = 1.0;
 #pragma unroll
           for (int e = 0; e < kMaxMom; ++e) {
-            if (e < nm) {
-              acc[e] += up;
-              if (e < ncols) acc[nm + e] += y * up;
-              up *= u;
-            }
+            acc[e] += up;
+            if (e < kMaxCols) acc[kMaxMom + e] += y * up;
+            up *= u;
           }
         }
       }
@@ -723,8 +749,8 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
       if (date > 1) {
 #pragma unroll
         for (int q = 0; q < kPpt; ++q) {
-          const uint64_t i = t0 + q * G;
-          nxt[q] = i < a.n ? spot_at<kFp32>(a.paths, a.n, date - 1, i) : 0.0;
+          const uint32_t i = t0 + q * G;
+          nxt[q] = i < a.n ? spot_raw<kFp32>(a.paths, a.n, date - 1, i) : SpotT(0);
         }
       }
     }
@@ -743,7 +769,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         const double mean = c + sc * mu;
         double sdv = sqrt(var) * sc;
         sdv = sdv > 1e-12 ? sdv : 1e-12;  // lsmc.rs:67-71
-        moments_x(tot, ncols, sc / sdv, (c - mean) / sdv, g_s, h_s);
+        moments_x(tot, ncols, kMaxMom, sc / sdv, (c - mean) / sdv, g_s, h_s);
         if (threadIdx.x == 0) {
           ms_s[0] = mean;
           ms_s[1] = sdv;
@@ -770,11 +796,13 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   double acc1[kMaxAccB] = {};
 #pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
-    const uint64_t i = t0 + q * G;
+    const uint32_t i = t0 + q * G;
     if (i < a.n) {
       double v = cf[q][tx];
       if (have_ex) {
-        const double s1 = kKeepSpot ? sp[kKeepSpot ? q : 0][tx] : spot_at<kFp32>(a.paths, a.n, 1, i);
+        const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
+                          : kKeepSpot ? sp[kKeepSpot ? q : 0][tx]
+                                      : spot_at<kFp32>(a.paths, a.n, 1, i);
         v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
       }
       cf[q][tx] = v * a.discount;
@@ -790,7 +818,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   double acc2[kMaxAccB] = {};
 #pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
-    const uint64_t i = t0 + q * G;
+    const uint32_t i = t0 + q * G;
     if (i < a.n) {
       const double dl = cf[q][tx] - mean;
       acc2[0] += dl * dl;
