@@ -116,13 +116,34 @@ __device__ __forceinline__ double spot_at(const void* paths, uint64_t n, uint64_
     return ((const double*)paths)[(date - 1) * n + i];
 }
 
-// The stored value itself (float for FP32 paths), for values kept in registers.
+// One date's column of the path store (n values) as a buffer resource: the
+// column base sits in SGPRs and each load takes a 32-bit byte offset, so the
+// per-path addresses of a thread cost no 64-bit registers; offsets at or past
+// n values read 0 (the hardware range check on num_records).
 template <bool kFp32>
-__device__ __forceinline__ typename std::conditional<kFp32, float, double>::type spot_raw(
-    const void* paths, uint64_t n, uint64_t date, uint32_t i) {
+struct SpotCol {
   using T = typename std::conditional<kFp32, float, double>::type;
-  const T* col = (const T*)paths + (date - 1) * n;  // wave-uniform column base
-  return col[i];
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ SpotCol(const void* paths, uint64_t n, uint64_t date)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)((const T*)paths + (date - 1) * n), 0,
+                                            (int)(n * sizeof(T)), 0x00020000)) {}
+  // value at byte offset `off` of the column
+  __device__ __forceinline__ T at(uint32_t off) const {
+    if constexpr (kFp32)
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    else
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  }
+  __device__ __forceinline__ T operator[](uint32_t i) const { return at(i * (uint32_t)sizeof(T)); }
+};
+
+// A thread's byte offset into a column, made opaque once per date so that the
+// per-path offsets (this plus q x the grid's stride) are formed at each load
+// instead of being kept live, 16 of them, across the whole induction.
+__device__ __forceinline__ uint32_t fresh_offset(uint32_t t0, uint32_t esz) {
+  uint32_t off = t0 * esz;
+  asm volatile("" : "+v"(off));
+  return off;
 }
 
 __device__ __forceinline__ double wave_sum_d(double v) {
@@ -682,14 +703,16 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   constexpr bool kKeepSpot = !kRegSpot && kPpt <= 8;
   constexpr int kUnrollQ = (kPpt <= 8 || kRegSpot) ? kPpt : 4;  // per-path loops
   using SpotT = typename std::conditional<kFp32, float, double>::type;
+  const uint32_t gstep = G * (uint32_t)sizeof(SpotT);  // bytes between a thread's paths
   __shared__ double cf[kPpt][kCoopBlock];
   __shared__ double sp[kKeepSpot ? kPpt : 1][kCoopBlock];
   SpotT cur[kRegSpot ? kPpt : 1];
   const int tx = threadIdx.x;
+  const SpotCol<kFp32> last(a.paths, a.n, a.steps);
 #pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
     const uint32_t i = t0 + q * G;
-    const SpotT r0 = i < a.n ? spot_raw<kFp32>(a.paths, a.n, a.steps, i) : SpotT(0);
+    const SpotT r0 = last[i];  // 0 past n
     const double s0 = (double)r0;
     if constexpr (kKeepSpot) sp[q][tx] = s0;
     if constexpr (kRegSpot) cur[q] = r0;
@@ -705,10 +728,11 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   constexpr bool kPrefetch = kPpt <= 4 || kRegSpot;
   SpotT nxt[kPrefetch ? kPpt : 1];
   if constexpr (kPrefetch) {
+    const SpotCol<kFp32> first(a.paths, a.n, a.steps - 1);  // steps >= 2
 #pragma unroll
     for (int q = 0; q < kPpt; ++q) {
       const uint32_t i = t0 + q * G;
-      nxt[q] = i < a.n ? spot_raw<kFp32>(a.paths, a.n, a.steps - 1, i) : SpotT(0);
+      nxt[q] = first[i];  // 0 past n
     }
   }
   for (uint32_t date = a.steps - 1; date >= 1; --date) {
@@ -716,6 +740,8 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
 #pragma unroll
     for (int e = 0; e < kMaxAccB; ++e) acc[e] = 0.0;
     const double inv_sc = 1.0 / sc;
+    const SpotCol<kFp32> col0(a.paths, a.n, date), col1(a.paths, a.n, date + 1);
+    const uint32_t toff = fresh_offset(t0, sizeof(SpotT));
 #pragma unroll kUnrollQ
     for (int q = 0; q < kPpt; ++q) {
       const uint32_t i = t0 + q * G;
@@ -724,13 +750,13 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         if (have_ex) {
           const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
                             : kKeepSpot ? sp[kKeepSpot ? q : 0][tx]
-                                        : spot_at<kFp32>(a.paths, a.n, date + 1, i);
+                                        : (double)col1.at(toff + q * gstep);
           v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
         }
         const double y = v * a.discount;  // lsmc.rs:52-54
         cf[q][tx] = y;
         const double s =
-            kPrefetch ? (double)nxt[kPrefetch ? q : 0] : spot_at<kFp32>(a.paths, a.n, date, i);
+            kPrefetch ? (double)nxt[kPrefetch ? q : 0] : (double)col0.at(toff + q * gstep);
         if constexpr (kKeepSpot) sp[q][tx] = s;
         if constexpr (kRegSpot) cur[q] = nxt[q];
         if (intrinsic(s, a.k, a.is_put) > 0.0) {
@@ -747,11 +773,10 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     }
     if constexpr (kPrefetch) {
       if (date > 1) {
+        const SpotCol<kFp32> colm(a.paths, a.n, date - 1);
+        const uint32_t toffm = fresh_offset(t0, sizeof(SpotT));
 #pragma unroll
-        for (int q = 0; q < kPpt; ++q) {
-          const uint32_t i = t0 + q * G;
-          nxt[q] = i < a.n ? spot_raw<kFp32>(a.paths, a.n, date - 1, i) : SpotT(0);
-        }
+        for (int q = 0; q < kPpt; ++q) nxt[q] = colm.at(toffm + q * gstep);  // 0 past n
       }
     }
     double* pp = a.part + (date & 1u) * pstride;
@@ -794,6 +819,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   }
   // exercise at date 1, final discount (lsmc.rs:102-104), sum
   double acc1[kMaxAccB] = {};
+  const SpotCol<kFp32> col1(a.paths, a.n, 1);
 #pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
     const uint32_t i = t0 + q * G;
@@ -802,7 +828,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
       if (have_ex) {
         const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
                           : kKeepSpot ? sp[kKeepSpot ? q : 0][tx]
-                                      : spot_at<kFp32>(a.paths, a.n, 1, i);
+                                      : (double)col1[i];
         v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
       }
       cf[q][tx] = v * a.discount;
