@@ -468,11 +468,6 @@ struct LsmcCoopArgs {
   unsigned* bar;  // grid barrier words (zeroed counters)
 };
 
-// C(e, j), e < kMaxMom
-__constant__ const double kBinom[kMaxMom][kMaxMom] = {
-    {1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}, {1, 5, 10, 10, 5, 1},
-    {1, 6, 15, 20, 15, 6, 1}, {1, 7, 21, 35, 35, 21, 7, 1}, {1, 8, 28, 56, 70, 56, 28, 8, 1}};
-
 // Moments of x = (S - mean) / sdv = al u + be from the raw moments of
 // u = (S - c) / sc (tot[q] = sum u^q, q < nm; tot[yoff + q] = sum cf u^q,
 // q < ncols): sum x^e = sum_q C(e, q) al^q be^(e-q) tot[q], and the same for
@@ -481,7 +476,8 @@ __constant__ const double kBinom[kMaxMom][kMaxMom] = {
 __device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff, double al,
                                           double be, double* g, double* h) {
   const int nm = 2 * (ncols - 1) + 1;
-  const int lane = threadIdx.x & 63;
+  int lane = threadIdx.x & 63;
+  asm volatile("" : "+v"(lane));  // lane-dependent values are formed here, not kept live
   const bool is_g = lane < nm, is_h = lane >= 16 && lane < 16 + ncols;
   if (!is_g && !is_h) return;
   const int e = is_g ? lane : lane - 16;
@@ -493,15 +489,18 @@ __device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff
     ap[r] = ap[r - 1] * al;
     bp[r] = bp[r - 1] * be;
   }
-  double s = 0.0;
+  double s = 0.0, binom = 1.0;  // binom = C(e, q)
 #pragma unroll
   for (int q = 0; q < kMaxMom; ++q) {
     if (q <= e) {
       double bq = 0.0;  // be^(e-q), selected without a dynamic register index
 #pragma unroll
       for (int r = 0; r < kMaxMom; ++r) bq = (r == e - q) ? bp[r] : bq;
-      s += kBinom[e][q] * ap[q] * bq * src[q];
+      s += binom * ap[q] * bq * src[q];
     }
+    // C(e, q+1) = C(e, q) (e - q) / (q + 1): the product is an exact integer
+    // below 2^10 and the rounded quotient lies within 1e-12 of the integer.
+    binom = rint(binom * (double)(e - q) * (1.0 / (q + 1)));
   }
   if (is_g)
     g[e] = s;
@@ -631,25 +630,28 @@ __device__ __forceinline__ double coop_block_sum(const double* acc, int na,
   return s;
 }
 
-// Every block: fixed-order total over blocks of part[b][0..na) -> tot (LDS).
-// Thread t adds blocks t, t+1024, ...; with one block per CU (<= 1024 blocks)
-// the partials sit in the first ceil(grid / 64) waves.
-__device__ __forceinline__ void coop_total(const double* part, int na,
-                                           double (*red)[kMaxAccB], double* tot) {
-  double acc[kMaxAccB];
+// Every block: fixed-order total over blocks of part[e * nb + b] (element e
+// of block b's partial, e < na <= 16) -> tot (LDS).  Wave e sums element e:
+// lane l adds blocks l, l + 64, ... (coalesced loads), then the lanes in a
+// fixed butterfly; one block barrier.
+__device__ __forceinline__ void coop_total(const double* part, int na, double* tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned nb = gridDim.x;
+  if (wave < na) {
+    const double* col = part + (uint64_t)wave * nb;
+    double s = 0.0;
+    for (unsigned b0 = 0; b0 < nb; b0 += 256) {  // 4 loads in flight per lane
+      double v[4];
 #pragma unroll
-  for (int e = 0; e < kMaxAccB; ++e) acc[e] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kCoopBlock) {
-#pragma unroll
-    for (int e = 0; e < kMaxAccB; ++e)
-      if (e < na) acc[e] += ld_agent(&part[(uint64_t)b * na + e]);
+      for (int k = 0; k < 4; ++k) {
+        const unsigned b = b0 + 64 * k + lane;
+        v[k] = b < nb ? ld_agent(&col[b]) : 0.0;
+      }
+      s += (v[0] + v[1]) + (v[2] + v[3]);
+    }
+    s = wave_sum_d(s);
+    if (lane == 0) tot[wave] = s;
   }
-  double s;
-  if (gridDim.x <= 256)
-    s = coop_block_sum<4>(acc, na, red);
-  else
-    s = coop_block_sum<kCoopWaves>(acc, na, red);
-  if (threadIdx.x < na) tot[threadIdx.x] = s;
   __syncthreads();
 }
 
@@ -781,10 +783,10 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     }
     double* pp = a.part + (date & 1u) * pstride;
     const double bs = coop_block_sum<kCoopWaves>(acc, na, red);
-    if (threadIdx.x < na) st_agent(&pp[(uint64_t)blockIdx.x * na + threadIdx.x], bs);
+    if (threadIdx.x < na) st_agent(&pp[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x], bs);
     grid_barrier(a.bar, gen);
-    coop_total(pp, na, red, tot);
-    if (threadIdx.x < 64) {  // wave 0: statistics and the moments of x
+    coop_total(pp, na, tot);
+    if (threadIdx.x < 64) {  // wave 0: statistics, the moments of x, the solve
       const double m = tot[0];
       const int skip = (uint64_t)m < (uint64_t)ncols + 1;  // lsmc.rs:62-64
       if (!skip) {
@@ -801,9 +803,15 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         }
       }
       if (threadIdx.x == 0) skip_s = skip;
+      if (!skip) {
+        // g_s and h_s come from this wave's own lanes: one wave's LDS
+        // operations complete in issue order, and the fence keeps the
+        // compiler from moving the solve's reads above those writes.
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (threadIdx.x == 0) solve_normal_eq(g_s, h_s, ncols, coef_s);
+      }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && !skip_s) solve_normal_eq(g_s, h_s, ncols, coef_s);
     __syncthreads();
     have_ex = skip_s == 0;  // the same in every block
     if (have_ex) {
@@ -839,7 +847,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   double s1 = coop_block_sum<kCoopWaves>(acc1, 1, red);
   if (threadIdx.x == 0) st_agent(&a.part[blockIdx.x], s1);
   grid_barrier(a.bar, gen);
-  coop_total(a.part, 1, red, tot);
+  coop_total(a.part, 1, tot);
   const double mean = tot[0] / (double)a.n;
   double acc2[kMaxAccB] = {};
 #pragma unroll kUnrollQ
@@ -856,7 +864,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   const double sum_cf = tot[0];
   grid_barrier(a.bar, gen);
   if (blockIdx.x == 0) {
-    coop_total(a.part + pstride, 1, red, tot);
+    coop_total(a.part + pstride, 1, tot);
     if (threadIdx.x == 0) {
       a.out[0] = sum_cf;
       a.out[1] = tot[0];
