@@ -83,9 +83,7 @@ struct DevBuf {
 // per device suffices.  Buffers grow on demand and live until process exit.
 //   dev     device scratch (block partials, results, paths)
 //   counter kCounters zeroed uint32s: [0] the last-block reductions of the
-//           fused kernels, [kBarrierSlot..] the grid barrier of the
-//           cooperative kernels; every kernel leaves them as it found them
-//           (the barrier's generation word only grows)
+//           fused kernels; every kernel leaves them as it found them
 //   host    pinned host memory for results copied back
 // A loud error when no device is visible (the library has no CPU path).
 inline int check_device(const Err& err) {
@@ -105,7 +103,6 @@ struct Workspace {
   size_t host_bytes = 0;
 };
 constexpr int kCounters = 1024;
-constexpr int kBarrierSlot = 256;
 // Workspace of the current device with at least the given sizes.
 int workspace(size_t dev_bytes, size_t host_bytes, Workspace** out, const Err& err);
 

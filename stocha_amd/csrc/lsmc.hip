@@ -77,7 +77,13 @@ __global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, c
                                                             const uint64_t n, const uint32_t steps,
                                                             void* __restrict__ paths,
                                                             double* __restrict__ cf, const double k,
-                                                            const int is_put) {
+                                                            const int is_put,
+                                                            unsigned* __restrict__ flags,
+                                                            const unsigned n_flags) {
+  // the cooperative induction that follows starts from zero barrier flags
+  if (blockIdx.x == 0)
+    for (unsigned j = threadIdx.x; j < n_flags; j += kLsBlock)
+      __hip_atomic_store(flags + j * 4, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   using Src = GbmSrc<kFp32>;
   using Model = typename std::conditional<kFp32, Gbm32, Gbm64>::type;
   constexpr uint32_t kSlots = kFp32 ? 16 : 8;
@@ -425,36 +431,39 @@ __device__ __forceinline__ void st_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid barrier for a co-resident grid (cooperative launch).  bar: 8 group
-// counters (64 bytes apart), a top counter at [128], the generation at
-// [144]; all counters are zero between barriers.  Blocks arrive at their
-// group counter (block % 8); the last of a group arrives at the top counter;
-// the last group bumps the generation, on which every block's thread 0
-// waits.  gen: the generation this block is in (read at kernel start).
-__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned& gen) {
+// Grid barrier for a co-resident grid (cooperative launch).  flags: one
+// 16-byte slot per block, zero when the kernel starts (the path generation
+// kernel before it clears them).  Barrier k (1, 2, ...) of the launch: each
+// block stores k in its slot, once its partial stores have completed, and
+// wave 0 of each block reads every slot until all hold at least k.  Plain
+// stores to separate slots: no atomic queues on one address, and arriving
+// and release are one store and one load apart.
+__device__ __forceinline__ void grid_barrier(unsigned* flags, unsigned& k) {
   __builtin_amdgcn_s_waitcnt(0);  // this wave's partial stores have completed
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned nb = gridDim.x;
-    const unsigned ng = nb < 8u ? nb : 8u;
-    const unsigned grp = blockIdx.x % ng;
-    const unsigned per = (nb - grp + ng - 1) / ng;
-    unsigned* cx = bar + grp * 16;
-    unsigned* ct = bar + 128;
-    unsigned* gp = bar + 144;
-    const unsigned g = gen;
-    if (atomicAdd(cx, 1u) == per - 1) {
-      st_agent(cx, 0u);
-      if (atomicAdd(ct, 1u) == ng - 1) {
-        st_agent(ct, 0u);
-        __builtin_amdgcn_s_waitcnt(0);
-        st_agent(gp, g + 1);
+  k = k + 1;
+  if (threadIdx.x < 64) {
+    const unsigned nb = gridDim.x, lane = threadIdx.x;
+    if (lane == 0) st_agent(flags + blockIdx.x * 4, k);
+    for (;;) {
+      bool wait = false;
+      for (unsigned b0 = 0; b0 < nb; b0 += 256) {  // 4 loads in flight per lane
+        unsigned v[4];
+        bool in[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {  // unpredicated loads (slot 0 past nb)
+          const unsigned b = b0 + 64 * j + lane;
+          in[j] = b < nb;
+          v[j] = ld_agent(flags + (in[j] ? b : 0u) * 4);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wait = wait | (in[j] & (v[j] < k));
       }
+      if (__builtin_amdgcn_ballot_w64(wait) == 0) break;
+      __builtin_amdgcn_s_sleep(1);
     }
-    while (ld_agent(gp) == g) __builtin_amdgcn_s_sleep(1);
   }
   __syncthreads();
-  gen = gen + 1;
 }
 
 struct LsmcCoopArgs {
@@ -465,7 +474,7 @@ struct LsmcCoopArgs {
   double k, discount;
   double* part;   // [2][grid][kMaxAccB]
   double* out;    // [2]: sum cf, sum (cf - mean)^2
-  unsigned* bar;  // grid barrier words (zeroed counters)
+  unsigned* bar;  // grid barrier flags (16 bytes per block), zero at kernel start
 };
 
 // Moments of x = (S - mean) / sdv = al u + be from the raw moments of
@@ -606,6 +615,15 @@ __device__ __forceinline__ double wave_reduce_scatter16(const double (&v)[16]) {
   return r;
 }
 
+// Block barrier ordering LDS only: unlike __syncthreads it does not wait for
+// this wave's outstanding global loads, so loads issued earlier stay in
+// flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Block sums of acc[0..na) over the first kWaves waves (the other waves hold
 // nothing): per-wave reduce-scatter, then the waves in order; thread a < na
 // returns slot a's total (red is free again on return).  Called by every
@@ -622,11 +640,11 @@ __device__ __forceinline__ double coop_block_sum(const double* acc, int na,
     const int slot = lane >> 2;
     if ((lane & 3) == 0 && slot < na) red[wave][slot] = r;
   }
-  __syncthreads();
+  lds_barrier();
   double s = 0.0;
   if (threadIdx.x < na)
     for (int w = 0; w < kWaves; ++w) s += red[w][threadIdx.x];
-  __syncthreads();
+  lds_barrier();
   return s;
 }
 
@@ -652,7 +670,7 @@ __device__ __forceinline__ void coop_total(const double* part, int na, double* t
     s = wave_sum_d(s);
     if (lane == 0) tot[wave] = s;
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // Exercise decision of lsmc.rs:91-98 with the coefficients of the last
@@ -678,7 +696,7 @@ __device__ __forceinline__ double exercise(double cf, double s, double k, int is
 
 template <bool kFp32, int kPpt>
 __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArgs a) {
-  unsigned gen = ld_agent(a.bar + 144);
+  unsigned gen = 0;  // grid barriers passed
   __shared__ double red[kCoopWaves][kMaxAccB];
   __shared__ double tot[kMaxAccB];
   __shared__ double coef_s[kMaxCols];
@@ -744,6 +762,9 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     const double inv_sc = 1.0 / sc;
     const SpotCol<kFp32> col0(a.paths, a.n, date), col1(a.paths, a.n, date + 1);
     const uint32_t toff = fresh_offset(t0, sizeof(SpotT));
+    // exercise at date+1 with its regression, then one discount step
+    // (lsmc.rs:91-98, 52-54); the spots of date are not needed yet, so
+    // their prefetched loads are still landing
 #pragma unroll kUnrollQ
     for (int q = 0; q < kPpt; ++q) {
       const uint32_t i = t0 + q * G;
@@ -755,8 +776,15 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
                                         : (double)col1.at(toff + q * gstep);
           v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
         }
-        const double y = v * a.discount;  // lsmc.rs:52-54
-        cf[q][tx] = y;
+        cf[q][tx] = v * a.discount;
+      }
+    }
+    // moments of the in-the-money spots of date (u basis)
+#pragma unroll kUnrollQ
+    for (int q = 0; q < kPpt; ++q) {
+      const uint32_t i = t0 + q * G;
+      if (i < a.n) {
+        const double y = cf[q][tx];
         const double s =
             kPrefetch ? (double)nxt[kPrefetch ? q : 0] : (double)col0.at(toff + q * gstep);
         if constexpr (kKeepSpot) sp[q][tx] = s;
@@ -773,6 +801,14 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         }
       }
     }
+    double* pp = a.part + (date & 1u) * pstride;
+    const double bs = coop_block_sum<kCoopWaves>(acc, na, red);
+    if (threadIdx.x < na) st_agent(&pp[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x], bs);
+    grid_barrier(a.bar, gen);
+    coop_total(pp, na, tot);
+    // The spots of date-1 are loaded here: after the totals' loads (vector
+    // memory returns in order), and ahead of the solve and the next exercise
+    // pass, which wait on LDS only.
     if constexpr (kPrefetch) {
       if (date > 1) {
         const SpotCol<kFp32> colm(a.paths, a.n, date - 1);
@@ -781,11 +817,6 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         for (int q = 0; q < kPpt; ++q) nxt[q] = colm.at(toffm + q * gstep);  // 0 past n
       }
     }
-    double* pp = a.part + (date & 1u) * pstride;
-    const double bs = coop_block_sum<kCoopWaves>(acc, na, red);
-    if (threadIdx.x < na) st_agent(&pp[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x], bs);
-    grid_barrier(a.bar, gen);
-    coop_total(pp, na, tot);
     if (threadIdx.x < 64) {  // wave 0: statistics, the moments of x, the solve
       const double m = tot[0];
       const int skip = (uint64_t)m < (uint64_t)ncols + 1;  // lsmc.rs:62-64
@@ -812,7 +843,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         if (threadIdx.x == 0) solve_normal_eq(g_s, h_s, ncols, coef_s);
       }
     }
-    __syncthreads();
+    lds_barrier();
     have_ex = skip_s == 0;  // the same in every block
     if (have_ex) {
       ex_mean = ms_s[0];
@@ -920,8 +951,11 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
   const size_t o_part = o_cf + up(sizeof(double) * n);
   const size_t o_ds = o_part + up(sizeof(double) * part_n);
   const size_t o_scal = o_ds + up(sizeof(DateState) * (steps + 1));
+  const size_t o_flag = o_scal + 256;
+  const unsigned n_flags = coop ? (unsigned)cus : 0u;
   Workspace* ws = nullptr;
-  if (int rc = workspace(o_scal + 256, 2 * sizeof(double), &ws, err)) return rc;
+  if (int rc = workspace(o_flag + up(16 * (size_t)n_flags), 2 * sizeof(double), &ws, err))
+    return rc;
   char* base = static_cast<char*>(ws->dev);
   void* paths = base;
   double* cf = reinterpret_cast<double*>(base + o_cf);
@@ -931,8 +965,9 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
 
   KernelTimer timer(st);
   timer.start();
+  unsigned* flags = reinterpret_cast<unsigned*>(base + o_flag);
   lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, (uint32_t)steps, paths, cf, k,
-                                                  is_put);
+                                                  is_put, flags, n_flags);
   STOCHA_HIP(hipGetLastError());
   if (coop) {
     LsmcCoopArgs ca;
@@ -945,7 +980,7 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
     ca.discount = discount;
     ca.part = part;
     ca.out = scal;
-    ca.bar = ws->counter + kBarrierSlot;
+    ca.bar = flags;
     if (int rc = lsmc_coop<kFp32>(ca, (unsigned)cus, st, err)) return rc;
     STOCHA_HIP(hipGetLastError());
   } else {
