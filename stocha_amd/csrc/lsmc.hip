@@ -481,7 +481,9 @@ struct LsmcCoopArgs {
 // u = (S - c) / sc (tot[q] = sum u^q, q < nm; tot[yoff + q] = sum cf u^q,
 // q < ncols): sum x^e = sum_q C(e, q) al^q be^(e-q) tot[q], and the same for
 // sum cf x^e with tot[yoff + q].  Lane e < nm of the calling wave writes g[e],
-// lane 16 + e (e < ncols) writes h[e].
+// lane 16 + e (e < ncols) writes h[e].  Per lane the sum is a Horner scheme
+// in be over q = 0..e (a few instructions per q on the statistics wave's
+// serial path).
 __device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff, double al,
                                           double be, double* g, double* h) {
   const int nm = 2 * (ncols - 1) + 1;
@@ -491,25 +493,15 @@ __device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff
   if (!is_g && !is_h) return;
   const int e = is_g ? lane : lane - 16;
   const double* src = is_g ? tot : tot + yoff;
-  double ap[kMaxMom], bp[kMaxMom];
-  ap[0] = bp[0] = 1.0;
-#pragma unroll
-  for (int r = 1; r < kMaxMom; ++r) {
-    ap[r] = ap[r - 1] * al;
-    bp[r] = bp[r - 1] * be;
-  }
-  double s = 0.0, binom = 1.0;  // binom = C(e, q)
+  double s = 0.0, binom = 1.0, aq = 1.0;  // binom = C(e, q), aq = al^q
 #pragma unroll
   for (int q = 0; q < kMaxMom; ++q) {
-    if (q <= e) {
-      double bq = 0.0;  // be^(e-q), selected without a dynamic register index
-#pragma unroll
-      for (int r = 0; r < kMaxMom; ++r) bq = (r == e - q) ? bp[r] : bq;
-      s += binom * ap[q] * bq * src[q];
-    }
+    const double term = binom * aq * src[q <= e ? q : 0];  // reads stay below nm / ncols
+    s = q <= e ? s * be + term : s;
     // C(e, q+1) = C(e, q) (e - q) / (q + 1): the product is an exact integer
     // below 2^10 and the rounded quotient lies within 1e-12 of the integer.
     binom = rint(binom * (double)(e - q) * (1.0 / (q + 1)));
+    aq *= al;
   }
   if (is_g)
     g[e] = s;
@@ -517,47 +509,55 @@ __device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff
     h[e] = s;
 }
 
-// G c = h with G[i][j] = g[i + j] (i, j < ncols) by Cholesky in FP64: the
-// least-squares minimiser of lsmc.rs:73-84 via the normal equations.  One
-// thread; loops run to kMaxCols with guards so the factor stays in registers.
-__device__ __forceinline__ void solve_normal_eq(const double* g, const double* h, int ncols,
-                                                double* coeff) {
-  double L[kMaxCols][kMaxCols], rd[kMaxCols], w[kMaxCols], c[kMaxCols];
+// A wave-uniform double moved to scalar registers.
+__device__ __forceinline__ double uniform_d(double v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// G c = h with G[i][j] = g[i + j] (i, j < ncols) in FP64: the least-squares
+// minimiser of lsmc.rs:73-84 via the normal equations, solved by the calling
+// wave (all 64 lanes).  Lane i holds row i of [G | h] (rows ncols..4 are
+// identity rows).  Gauss-Jordan elimination without pivoting: step k
+// broadcasts pivot row k with readlane and every other row subtracts its
+// multiple of it, so afterwards lane i holds c_i = h_i / G_ii on its own and
+// writes it.  G is SPD when the in-the-money spots take at least ncols
+// values, so every pivot is positive; a pivot <= 0 marks a rank-deficient
+// basis, and then the regression falls back to the constant.
+__device__ __forceinline__ void solve_normal_eq_wave(const double* g, const double* h, int ncols,
+                                                     double* coeff) {
+  int lane = threadIdx.x & 63;
+  asm volatile("" : "+v"(lane));  // lane-dependent values are formed here, not kept live
+  const bool row = lane < ncols;
+  double r[kMaxCols], hh = row ? h[lane] : 0.0;
+#pragma unroll
+  for (int j = 0; j < kMaxCols; ++j)
+    r[j] = (row && j < ncols) ? g[(row ? lane : 0) + j] : (lane == j ? 1.0 : 0.0);
   bool ok = true;
+  double my_inv = 1.0;  // lane i: 1 / (pivot i)
 #pragma unroll
-  for (int i = 0; i < kMaxCols; ++i) {
+  for (int k = 0; k < kMaxCols; ++k) {
+    double pk[kMaxCols];
 #pragma unroll
-    for (int j = 0; j <= i; ++j) {
-      double v = (i < ncols) ? g[i + j] : (i == j ? 1.0 : 0.0);
+    for (int j = k; j < kMaxCols; ++j) pk[j] = readlane_d(r[j], k);
+    const double phk = readlane_d(hh, k);
+    if (k < ncols) ok = ok && pk[k] > 0.0;
+    const double inv = 1.0 / pk[k];
+    my_inv = lane == k ? inv : my_inv;
+    const double f = lane != k ? r[k] * inv : 0.0;
 #pragma unroll
-      for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
-      if (i == j) {
-        if (i < ncols) ok = ok && v > 0.0;
-        L[i][i] = sqrt(v > 0.0 ? v : 1.0);
-        rd[i] = 1.0 / L[i][i];
-      } else {
-        L[i][j] = v * rd[j];
-      }
-    }
+    for (int j = k + 1; j < kMaxCols; ++j) r[j] -= f * pk[j];
+    hh -= f * phk;
   }
-#pragma unroll
-  for (int i = 0; i < kMaxCols; ++i) {
-    double v = i < ncols ? h[i] : 0.0;
-#pragma unroll
-    for (int q = 0; q < i; ++q) v -= L[i][q] * w[q];
-    w[i] = v * rd[i];
-  }
-#pragma unroll
-  for (int i = kMaxCols - 1; i >= 0; --i) {
-    double v = w[i];
-#pragma unroll
-    for (int q = i + 1; q < kMaxCols; ++q)
-      if (q < ncols) v -= L[q][i] * c[q];
-    c[i] = v * rd[i];
-  }
-#pragma unroll
-  for (int i = 0; i < kMaxCols; ++i) coeff[i] = (ok && i < ncols) ? c[i] : 0.0;
-  if (!ok) coeff[0] = h[0] / g[0];  // rank-deficient basis: regress on the constant
+  if (lane < kMaxCols) coeff[lane] = (ok && row) ? hh * my_inv : 0.0;
+  if (!ok && lane == 0) coeff[0] = h[0] / g[0];  // rank-deficient basis: the constant
 }
 
 // ---- wave reduce-scatter of 16 FP64 slots (gfx950 lane exchanges) ----
@@ -707,6 +707,11 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   // and a spot's address is a per-date column base plus a 32-bit offset.
   const uint32_t G = gridDim.x * kCoopBlock;
   const uint32_t t0 = blockIdx.x * kCoopBlock + threadIdx.x;
+  // kPpt is the smallest power of two with kPpt x G >= n, so n > kPpt / 2 x G
+  // and the first half of a thread's paths always exist: only the second
+  // half needs the per-lane check (fewer live predicates).
+  const uint64_t n_paths = a.n;
+  auto live = [n_paths](int q, uint32_t i) { return q < kPpt / 2 || i < n_paths; };
   // Accumulators in a fixed layout: [0, kMaxMom) sum u^e and
   // [kMaxMom, kMaxAccB) sum cf u^e, all of them for every degree (static
   // register indices, no branches on the degree); the solve reads the first
@@ -736,7 +741,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     const double s0 = (double)r0;
     if constexpr (kKeepSpot) sp[q][tx] = s0;
     if constexpr (kRegSpot) cur[q] = r0;
-    cf[q][tx] = i < a.n ? intrinsic(s0, a.k, a.is_put) : 0.0;  // lsmc.rs:45-47
+    cf[q][tx] = live(q, i) ? intrinsic(s0, a.k, a.is_put) : 0.0;  // lsmc.rs:45-47
   }
   bool have_ex = false;  // a regression was solved at date+1
   double ex_mean = 0.0, ex_std = 1.0, ex_inv = 1.0, ex_c[kMaxCols];
@@ -759,7 +764,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     double acc[kMaxAccB];
 #pragma unroll
     for (int e = 0; e < kMaxAccB; ++e) acc[e] = 0.0;
-    const double inv_sc = 1.0 / sc;
+    const double inv_sc = uniform_d(1.0 / sc);
     const SpotCol<kFp32> col0(a.paths, a.n, date), col1(a.paths, a.n, date + 1);
     const uint32_t toff = fresh_offset(t0, sizeof(SpotT));
     // exercise at date+1 with its regression, then one discount step
@@ -768,7 +773,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
 #pragma unroll kUnrollQ
     for (int q = 0; q < kPpt; ++q) {
       const uint32_t i = t0 + q * G;
-      if (i < a.n) {
+      if (live(q, i)) {
         double v = cf[q][tx];
         if (have_ex) {
           const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
@@ -783,7 +788,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
 #pragma unroll kUnrollQ
     for (int q = 0; q < kPpt; ++q) {
       const uint32_t i = t0 + q * G;
-      if (i < a.n) {
+      if (live(q, i)) {
         const double y = cf[q][tx];
         const double s =
             kPrefetch ? (double)nxt[kPrefetch ? q : 0] : (double)col0.at(toff + q * gstep);
@@ -840,17 +845,17 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         // compiler from moving the solve's reads above those writes.
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (threadIdx.x == 0) solve_normal_eq(g_s, h_s, ncols, coef_s);
+        solve_normal_eq_wave(g_s, h_s, ncols, coef_s);
       }
     }
     lds_barrier();
-    have_ex = skip_s == 0;  // the same in every block
-    if (have_ex) {
-      ex_mean = ms_s[0];
-      ex_std = ms_s[1];
-      ex_inv = 1.0 / ex_std;
+    have_ex = __builtin_amdgcn_readfirstlane(skip_s) == 0;  // the same in every block
+    if (have_ex) {  // block-uniform values, held in scalar registers
+      ex_mean = uniform_d(ms_s[0]);
+      ex_std = uniform_d(ms_s[1]);
+      ex_inv = uniform_d(1.0 / ex_std);
 #pragma unroll
-      for (int j = 0; j < kMaxCols; ++j) ex_c[j] = coef_s[j];
+      for (int j = 0; j < kMaxCols; ++j) ex_c[j] = uniform_d(coef_s[j]);
       c = ex_mean;
       sc = ex_std;
     }
@@ -862,7 +867,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
 #pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
     const uint32_t i = t0 + q * G;
-    if (i < a.n) {
+    if (live(q, i)) {
       double v = cf[q][tx];
       if (have_ex) {
         const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
@@ -884,7 +889,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
 #pragma unroll kUnrollQ
   for (int q = 0; q < kPpt; ++q) {
     const uint32_t i = t0 + q * G;
-    if (i < a.n) {
+    if (live(q, i)) {
       const double dl = cf[q][tx] - mean;
       acc2[0] += dl * dl;
     }
