@@ -673,22 +673,28 @@ __device__ __forceinline__ void coop_total(const double* part, int na, double* t
   lds_barrier();
 }
 
+// Payoff max(sgn (s - k), 0) with sgn = -1 for a put, +1 for a call, and
+// nsk = -sgn k: the fused multiply-add rounds once, so the value is k - s or
+// s - k exactly as lsmc.rs:25-31 forms it.
+__device__ __forceinline__ double intrinsic_sgn(double s, double sgn, double nsk) {
+  const double v = __builtin_fma(sgn, s, nsk);
+  return v > 0.0 ? v : 0.0;
+}
+
 // Exercise decision of lsmc.rs:91-98 with the coefficients of the last
-// regression date; returns the (undiscounted) cash flow.
-// (x is formed with the reciprocal of the std, one multiply instead of a
-// division per path; the regression coefficients already differ from the
-// reference's QR in the last bits, so this changes nothing measurable.)
-__device__ __forceinline__ double exercise(double cf, double s, double k, int is_put, int ncols,
+// regression date (c[j] = 0 for j >= ncols); returns the (undiscounted) cash
+// flow.  x is formed with the reciprocal of the std and the continuation
+// value by Horner's rule in fused multiply-adds instead of sum c_j x^j: both
+// move the value by a few ulps, below the difference between the normal
+// equations here and the reference's QR, so no decision moves measurably.
+__device__ __forceinline__ double exercise(double cf, double s, double sgn, double nsk,
                                            double mean, double inv_sdv, const double* c) {
-  const double exv = intrinsic(s, k, is_put);
+  const double exv = intrinsic_sgn(s, sgn, nsk);
   if (exv > 0.0) {
     const double x = (s - mean) * inv_sdv;
-    double cont = 0.0;
+    double cont = c[kMaxCols - 1];
 #pragma unroll
-    for (int j = 0; j < kMaxCols; ++j) {
-      const double t = c[j] * powi_rt(x, j);
-      cont += j < ncols ? t : 0.0;  // a select: adding +0.0 changes no decision
-    }
+    for (int j = kMaxCols - 2; j >= 0; --j) cont = __builtin_fma(cont, x, c[j]);
     if (exv > (cont > 0.0 ? cont : 0.0)) return exv;
   }
   return cf;
@@ -717,6 +723,8 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   // register indices, no branches on the degree); the solve reads the first
   // 2 deg + 1 and deg + 1 of them.
   const int ncols = a.ncols, na = kMaxAccB;
+  const bool deg4 = ncols == kMaxCols;
+  const double sgn = a.is_put ? -1.0 : 1.0, nsk = -sgn * a.k;
   const uint64_t pstride = (uint64_t)gridDim.x * na;
   // Cash flows live in LDS (kPpt x 1024 doubles, <= 128 KB with one block
   // per CU).  The spots of date+1: FP32 paths above 4 per thread keep them
@@ -741,7 +749,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     const double s0 = (double)r0;
     if constexpr (kKeepSpot) sp[q][tx] = s0;
     if constexpr (kRegSpot) cur[q] = r0;
-    cf[q][tx] = live(q, i) ? intrinsic(s0, a.k, a.is_put) : 0.0;  // lsmc.rs:45-47
+    cf[q][tx] = live(q, i) ? intrinsic_sgn(s0, sgn, nsk) : 0.0;  // lsmc.rs:45-47
   }
   bool have_ex = false;  // a regression was solved at date+1
   double ex_mean = 0.0, ex_std = 1.0, ex_inv = 1.0, ex_c[kMaxCols];
@@ -779,7 +787,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
           const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
                             : kKeepSpot ? sp[kKeepSpot ? q : 0][tx]
                                         : (double)col1.at(toff + q * gstep);
-          v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
+          v = exercise(v, s1, sgn, nsk, ex_mean, ex_inv, ex_c);
         }
         cf[q][tx] = v * a.discount;
       }
@@ -794,14 +802,20 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
             kPrefetch ? (double)nxt[kPrefetch ? q : 0] : (double)col0.at(toff + q * gstep);
         if constexpr (kKeepSpot) sp[q][tx] = s;
         if constexpr (kRegSpot) cur[q] = nxt[q];
-        if (intrinsic(s, a.k, a.is_put) > 0.0) {
+        if (intrinsic_sgn(s, sgn, nsk) > 0.0) {
           const double u = (s - c) * inv_sc;
-          double up = 1.0;
+          double up = 1.0, u4 = 0.0;
 #pragma unroll
-          for (int e = 0; e < kMaxMom; ++e) {
+          for (int e = 0; e < kMaxMom - 2; ++e) {  // degree <= 3: u^0..u^6, cf u^0..u^3
             acc[e] += up;
-            if (e < kMaxCols) acc[kMaxMom + e] += y * up;
+            if (e < kMaxCols - 1) acc[kMaxMom + e] = __builtin_fma(y, up, acc[kMaxMom + e]);
+            if (e == kMaxCols - 1) u4 = up;
             up *= u;
+          }
+          if (deg4) {  // block-uniform: u^7, u^8 and cf u^4
+            acc[kMaxMom - 2] += up;
+            acc[kMaxMom - 1] += up * u;
+            acc[kMaxMom + kMaxCols - 1] = __builtin_fma(y, u4, acc[kMaxMom + kMaxCols - 1]);
           }
         }
       }
@@ -873,7 +887,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         const double s1 = kRegSpot    ? (double)cur[kRegSpot ? q : 0]
                           : kKeepSpot ? sp[kKeepSpot ? q : 0][tx]
                                       : (double)col1[i];
-        v = exercise(v, s1, a.k, a.is_put, ncols, ex_mean, ex_inv, ex_c);
+        v = exercise(v, s1, sgn, nsk, ex_mean, ex_inv, ex_c);
       }
       cf[q][tx] = v * a.discount;
       acc1[0] += cf[q][tx];
