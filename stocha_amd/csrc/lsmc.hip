@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, c
     hook.s0f = (float)mc.s0;
     Pcg g{stream_at(&sd, hook.valid ? i : 0)};
     (void)queue_path<Src, Model, kSlots, kLsBlock>(mc, g, sd.inc, steps, &zig, q, false, hook);
-    if (hook.valid) {
+    if (cf != nullptr && hook.valid) {
       double last;
       if constexpr (kFp32)
         last = (double)((const float*)paths)[(uint64_t)(steps - 1) * n + i];
@@ -985,8 +985,10 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
   KernelTimer timer(st);
   timer.start();
   unsigned* flags = reinterpret_cast<unsigned*>(base + o_flag);
-  lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, (uint32_t)steps, paths, cf, k,
-                                                  is_put, flags, n_flags);
+  // the cooperative induction takes its cash flows from the last column
+  // itself (no cf array)
+  lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, (uint32_t)steps, paths,
+                                                  coop ? nullptr : cf, k, is_put, flags, n_flags);
   STOCHA_HIP(hipGetLastError());
   if (coop) {
     LsmcCoopArgs ca;
