@@ -20,6 +20,7 @@
  *   stocha_rng_new / _uniform / _normal <- src/lib.rs:75-145 (class RNG)
  *   stocha_hull_white            <- src/lib.rs:606-658   (hull_white)
  *   stocha_gaussian_copula       <- src/lib.rs:538-558   (gaussian_copula)
+ *   stocha_student_t_copula      <- src/lib.rs:563-578   (student_t_copula)
  *
  * Conventions (all entry points):
  *   - Plain C types only.  Integer arguments the reference takes as usize/u64
@@ -285,6 +286,17 @@ int stocha_hull_white(double r0, double a, double theta, double sigma, double t,
  * out == NULL validates only. */
 int stocha_gaussian_copula(const double *corr, uint64_t rows, uint64_t cols, uint64_t n_samples,
                            uint64_t seed, double *out, int where, char *err, size_t errlen);
+
+/* src/lib.rs:563-578 -> copula/student_t.rs:135-169: n_samples rows of
+ * t_nu((L z) sqrt(nu / chi2)), per sample a chi-square(nu) (Marsaglia-Tsang
+ * gamma(nu/2), doubled) and then dim normals, all from ONE stream
+ * Pcg64Dxsm::new(seed) decoded in parallel; t_nu is the reference's
+ * incomplete-beta CDF.  Checks: nu > 2 ("nu must be > 2", also for a NaN nu,
+ * on which the reference's sampler would never return), square corr,
+ * cholesky(corr).  out: (n_samples, dim) f64; out == NULL validates only. */
+int stocha_student_t_copula(const double *corr, uint64_t rows, uint64_t cols, double nu,
+                            uint64_t n_samples, uint64_t seed, double *out, int where, char *err,
+                            size_t errlen);
 
 /* ---- the RNG class (src/lib.rs:75-184): one sequential stream ------------
  * A generator is its 128-bit LCG state and increment, each as two uint64_t

@@ -79,6 +79,13 @@ def lib():
         L.orc_hull_white.argtypes = [_d] * 5 + [_u, _u, _u, _pd]
         L.orc_gaussian_copula.argtypes = [_pd, _u, _u, _u, _pd]
         L.orc_gaussian_copula.restype = None
+        L.orc_lgamma_lanczos.argtypes = [_d]
+        L.orc_lgamma_lanczos.restype = _d
+        L.orc_student_t_cdf.argtypes = [_d, _d]
+        L.orc_student_t_cdf.restype = _d
+        L.orc_student_t_copula.argtypes = [_pd, _u, _d, _u, _u, _pd, _pd, _pd,
+                                           C.POINTER(C.c_uint64)]
+        L.orc_student_t_copula.restype = None
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -340,6 +347,39 @@ def gaussian_copula(corr, n_samples, seed=42):
     L = np.ascontiguousarray(L, np.float64)
     out = np.empty((n_samples, d), np.float64)
     lib().orc_gaussian_copula(_p(L), d, n_samples, seed, _p(out))
+    return out
+
+
+def lgamma_lanczos(x):
+    """copula/student_t.rs:67-91 (Lanczos g = 7, 9 coefficients)."""
+    return lib().orc_lgamma_lanczos(float(x))
+
+
+def student_t_cdf(x, nu):
+    """copula/student_t.rs:8-63 (incomplete beta by modified Lentz); x scalar
+    or array."""
+    xs = np.asarray(x, np.float64)
+    f = lib().orc_student_t_cdf
+    out = np.array([f(float(v), float(nu)) for v in xs.ravel()], np.float64)
+    return out.reshape(xs.shape) if xs.shape else float(out[0])
+
+
+def student_t_copula(corr, nu, n_samples, seed=42, details=False):
+    """copula/student_t.rs:135-169 on the stream Pcg64Dxsm::new(seed).  With
+    details=True also returns the normals (n, d), the scales sqrt(nu / chi2)
+    (n,) and the draws consumed."""
+    corr = np.ascontiguousarray(corr, np.float64)
+    d = corr.shape[0]
+    L = cholesky(corr) if d else np.zeros((0, 0))
+    L = np.ascontiguousarray(L, np.float64)
+    out = np.empty((n_samples, d), np.float64)
+    z = np.empty((n_samples, d), np.float64)
+    s = np.empty(n_samples, np.float64)
+    used = C.c_uint64(0)
+    lib().orc_student_t_copula(_p(L), d, float(nu), n_samples, seed, _p(out), _p(z), _p(s),
+                               C.byref(used))
+    if details:
+        return out, z, s, used.value
     return out
 
 

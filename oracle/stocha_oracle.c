@@ -1187,3 +1187,120 @@ void orc_gaussian_copula(const double *L, uint64_t d, uint64_t n_samples, uint64
     }
     free(z);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Student-t copula: copula/student_t.rs on one stream new(seed).            */
+/* ------------------------------------------------------------------------ */
+/* student_t.rs:67-91: log-gamma, Lanczos g = 7 with 9 coefficients */
+double orc_lgamma_lanczos(double x) {
+    static const double C[9] = {0.99999999999980993,  676.5203681218851,     -1259.1392167224028,
+                                771.32342877765313,   -176.61502916214059,   12.507343278686905,
+                                -0.13857109526572012, 9.9843695780195716e-6, 1.5056327351493116e-7};
+    const double PI = 3.14159265358979323846;
+    if (x < 0.5) return log(PI) - log(sin(PI * x)) - orc_lgamma_lanczos(1.0 - x);
+    x -= 1.0;
+    double acc = C[0];
+    for (int i = 1; i < 9; i++) acc += C[i] / (x + (double)i);
+    double t = x + 7.0 + 0.5;
+    return 0.5 * log(2.0 * PI) + (x + 0.5) * log(t) - t + log(acc);
+}
+
+/* student_t.rs:32-63: continued fraction (modified Lentz), EPS 3e-7 */
+static double orc_beta_cf(double x, double a, double b) {
+    const double EPS = 3e-7, FPMIN = 1e-30;
+    double qab = a + b, qap = a + 1.0, qam = a - 1.0;
+    double c = 1.0;
+    double d = fmax(1.0 - qab * x / qap, FPMIN);
+    d = 1.0 / d;
+    double h = d;
+    for (int m = 1; m <= 200; m++) {
+        double mf = (double)m, m2 = 2.0 * mf;
+        double aa = mf * (b - mf) * x / ((qam + m2) * (a + m2));
+        d = fmax(fabs(1.0 + aa * d), FPMIN);
+        c = fmax(fabs(1.0 + aa / c), FPMIN);
+        d = 1.0 / d;
+        h *= d * c;
+        aa = -(a + mf) * (qab + mf) * x / ((a + m2) * (qap + m2));
+        d = fmax(fabs(1.0 + aa * d), FPMIN);
+        c = fmax(fabs(1.0 + aa / c), FPMIN);
+        d = 1.0 / d;
+        double del = d * c;
+        h *= del;
+        if (fabs(del - 1.0) < EPS) break;
+    }
+    return h;
+}
+
+/* student_t.rs:19-30: regularized incomplete beta I_x(a, b) */
+static double orc_ibeta(double x, double a, double b) {
+    if (x <= 0.0) return 0.0;
+    if (x >= 1.0) return 1.0;
+    if (x > (a + 1.0) / (a + b + 2.0)) return 1.0 - orc_ibeta(1.0 - x, b, a);
+    double log_beta = orc_lgamma_lanczos(a) + orc_lgamma_lanczos(b) - orc_lgamma_lanczos(a + b);
+    double front = exp(a * log(x) + b * log(1.0 - x) - log_beta) / a;
+    return front * orc_beta_cf(x, a, b);
+}
+
+/* student_t.rs:8-16 */
+double orc_student_t_cdf(double x, double nu) {
+    double z = nu / (nu + x * x);
+    double ib = orc_ibeta(z, nu / 2.0, 0.5);
+    return x >= 0.0 ? 1.0 - 0.5 * ib : 0.5 * ib;
+}
+
+/* student_t.rs:94-119: Marsaglia-Tsang gamma(shape), shape < 1 by the
+ * Ahrens-Dieter boost (a uniform first) */
+static double orc_gamma_mt(pcg_t *g, double shape) {
+    if (shape < 1.0) {
+        double u = pcg_next_f64(g);
+        return orc_gamma_mt(g, shape + 1.0) * pow(u, 1.0 / shape);
+    }
+    double d = shape - 1.0 / 3.0;
+    double c = 1.0 / sqrt(9.0 * d);
+    for (;;) {
+        double x = zig_normal(g);
+        double v_inner = 1.0 + c * x;
+        if (v_inner <= 0.0) continue;
+        double v = v_inner * v_inner * v_inner;
+        double u = pcg_next_f64(g);
+        if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v;
+        if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v;
+    }
+}
+
+/* student_t.rs:135-169: per sample a chi-square(nu) = 2 gamma(nu/2), then d
+ * normals; u_ij = t_cdf((L z_i)_j sqrt(nu / chi2), nu).  Also returns the
+ * stream draws consumed (*used) and, when z_out / s_out are given, the
+ * normals and the scale of every sample. */
+void orc_student_t_copula(const double *L, uint64_t d, double nu, uint64_t n_samples,
+                          uint64_t seed, double *out, double *z_out, double *s_out,
+                          uint64_t *used) {
+    pcg_t g;
+    pcg_new(&g, seed);
+    double *z = (double *)malloc(sizeof(double) * (d ? d : 1));
+    for (uint64_t i = 0; i < n_samples; i++) {
+        double chi_sq = 2.0 * orc_gamma_mt(&g, nu / 2.0);
+        double scale = sqrt(nu / chi_sq);
+        for (uint64_t k = 0; k < d; k++) z[k] = zig_normal(&g);
+        if (z_out)
+            for (uint64_t k = 0; k < d; k++) z_out[i * d + k] = z[k];
+        if (s_out) s_out[i] = scale;
+        for (uint64_t j = 0; j < d; j++) {
+            double xj = 0.0;
+            for (uint64_t k = 0; k <= j; k++) xj += L[j * d + k] * z[k];
+            out[i * d + j] = orc_student_t_cdf(xj * scale, nu);
+        }
+    }
+    free(z);
+    if (used) {
+        /* draws consumed: count the LCG steps from s0 to the final state */
+        pcg_t h;
+        pcg_new(&h, seed);
+        uint64_t k = 0;
+        while (h.state != g.state) {
+            h.state = h.state * (u128)PCG_CHEAP_MUL + h.inc;
+            k++;
+        }
+        *used = k;
+    }
+}

@@ -127,6 +127,13 @@ int orc_hull_white(double r0, double a, double theta, double sigma, double t, ui
 /* copula/gaussian.rs:15-73 given the Cholesky factor L (d x d). */
 void orc_gaussian_copula(const double *L, uint64_t d, uint64_t n_samples, uint64_t seed,
                          double *out);
+/* copula/student_t.rs: log-gamma (Lanczos), t CDF and the copula samples
+ * (z_out, s_out, used optional: the normals, scales and draws consumed). */
+double orc_lgamma_lanczos(double x);
+double orc_student_t_cdf(double x, double nu);
+void orc_student_t_copula(const double *L, uint64_t d, double nu, uint64_t n_samples,
+                          uint64_t seed, double *out, double *z_out, double *s_out,
+                          uint64_t *used);
 
 /* Number of OpenMP threads the oracle will use. */
 int orc_num_threads(void);

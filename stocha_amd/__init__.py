@@ -18,6 +18,7 @@ errors:
                             sequential stream, drawn on the GPU; JSON state)
     hull_white             (__init__.py:451-492)
     gaussian_copula        (__init__.py:400-423)
+    student_t_copula       (__init__.py:426-448)
 
 They run on the GPU through the C ABI of libstocha_gpu.so (hand-written HIP
 kernels); there is no CPU fallback.  Forms beside the drop-in API:
@@ -27,9 +28,9 @@ kernels); there is no CPU fallback.  Forms beside the drop-in API:
     basket_price           multi-asset basket call with pathwise adjoint Greeks
                            (stocha_amd.autograd wraps it as a torch.autograd.Function)
 
-The reference's other functions (sobol, halton, the Student-t copula, SABR,
-Heston COS / calibration, SSVI, var_cvar) are not
-part of this package: calling them raises NotImplementedError.
+The reference's other functions (sobol, halton, SABR, Heston COS /
+calibration, SSVI, var_cvar) are not part of this package: calling them
+raises NotImplementedError.
 """
 from __future__ import annotations
 
@@ -54,6 +55,7 @@ __all__ = [
     "RNG",
     "hull_white",
     "gaussian_copula",
+    "student_t_copula",
     "barrier_price",
     "asian_price",
     "lookback_price",
@@ -191,7 +193,15 @@ def hull_white(r0: float, a: float, theta: float, sigma: float, t: float, steps:
 def gaussian_copula(corr: np.ndarray, n_samples: int, seed: int = 42) -> np.ndarray:
     """Gaussian copula samples Phi(L z) from one PCG64DXSM stream; (n_samples,
     dim) in (0, 1) (reference: python/stocha/__init__.py:400-423)."""
-    return _S.gaussian_copula(corr=corr, n_samples=n_samples, seed=seed)
+    return _S.gaussian_copula(corr=np.asarray(corr, dtype=np.float64), n_samples=n_samples,
+                              seed=seed)
+
+
+def student_t_copula(corr: np.ndarray, nu: float, n_samples: int, seed: int = 42) -> np.ndarray:
+    """Student-t copula samples t_nu((L z) sqrt(nu / chi2)) from one PCG64DXSM
+    stream; (n_samples, dim) in (0, 1) (reference: python/stocha/__init__.py:426-448)."""
+    return _S.student_t_copula(corr=np.asarray(corr, dtype=np.float64), nu=nu,
+                               n_samples=n_samples, seed=seed)
 
 
 # ------------------------------------------------------------------ extras --
@@ -276,7 +286,7 @@ def _not_here(name):
     return f
 
 
-for _name in ("sobol", "halton", "var_cvar", "student_t_copula", "sabr_implied_vol", "sabr_calibrate", "heston_price",
+for _name in ("sobol", "halton", "var_cvar", "sabr_implied_vol", "sabr_calibrate", "heston_price",
               "heston_calibrate", "ssvi_calibrate", "ssvi_implied_vol", "ssvi_local_vol"):
     globals()[_name] = _not_here(_name)
 del _name

@@ -66,6 +66,7 @@ _SIGS = {
     "stocha_rng_normal": ([_pu, _pu, _u, _d, _d, _i, _vp, _i, _pu, _pu, _ps, _sz], C.c_int),
     "stocha_hull_white": ([_d] * 5 + [_u, _u, _u, _vp, _i, _ps, _sz], C.c_int),
     "stocha_gaussian_copula": ([_pd, _u, _u, _u, _u, _vp, _i, _ps, _sz], C.c_int),
+    "stocha_student_t_copula": ([_pd, _u, _u, _d, _u, _u, _vp, _i, _ps, _sz], C.c_int),
     "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _pd, _pd, _pd,
                                _ps, _sz], C.c_int),
     "stocha_lsmc_ex": ([_d] * 5 + [_u, _u, _i, _u, _u, _i, _pd, _pd, _ps, _sz], C.c_int),

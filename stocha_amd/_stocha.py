@@ -540,6 +540,24 @@ def hull_white(r0, a, theta, sigma, t, steps, n_paths, seed=42):
     return out
 
 
+def student_t_copula(corr, nu, n_samples, seed=42):
+    """src/lib.rs:563-578 (signature :562).  Returns (n_samples, dim) float64."""
+    if not isinstance(corr, np.ndarray) or corr.dtype != np.float64:
+        raise TypeError("argument 'corr': expected a numpy.ndarray of float64")
+    if corr.ndim != 2:
+        raise TypeError(f"argument 'corr': expected a 2-dimensional array, got {corr.ndim}")
+    nu = _f64(nu, "nu")
+    n_samples, seed = _uint(n_samples, "n_samples"), _uint(seed, "seed")
+    c = np.ascontiguousarray(corr)
+    lib, eb = L.load(), L.ErrBuf()
+    L.check(lib.stocha_student_t_copula(_pd(c), c.shape[0], c.shape[1], nu, n_samples, seed,
+                                        None, L.STOCHA_HOST, eb.buf, eb.n), eb)
+    out = np.empty((n_samples, c.shape[0]), dtype=np.float64)
+    L.check(lib.stocha_student_t_copula(_pd(c), c.shape[0], c.shape[1], nu, n_samples, seed,
+                                        out.ctypes.data, L.STOCHA_HOST, eb.buf, eb.n), eb)
+    return out
+
+
 def gaussian_copula(corr, n_samples, seed=42):
     """src/lib.rs:538-558 (signature :537).  Returns (n_samples, dim) float64."""
     if not isinstance(corr, np.ndarray) or corr.dtype != np.float64:

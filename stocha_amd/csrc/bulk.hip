@@ -337,6 +337,289 @@ __global__ __launch_bounds__(kBulkBlock) void gauss_copula_kernel(const double* 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Student-t copula (copula/student_t.rs:135-169) on one stream new(seed).
+// Sample i draws a chi-square(nu) = 2 gamma(nu/2) (Marsaglia-Tsang,
+// student_t.rs:94-119: a normal, a uniform unless 1 + c x <= 0, repeated
+// until accepted), then d normals.  Every sample starts in the same state,
+// so the draws a sample takes, len(p), depend only on its start position p.
+// The starts are found in parallel:
+//   1. draws:  the raw stream, positions [0, n_tot), into memory;
+//   2. len:    len(p) for every position p < n_pos (one sample decoded per
+//              position; kLenBad if it runs past n_tot);
+//   3. maps:   positions in chunks of kTChunk; per chunk and entry offset e
+//              (the first start in the chunk is at chunk + e) the exit
+//              offset into the next chunk and the number of starts;
+//   4. scan:   an inclusive scan composing the maps gives every chunk's true
+//              entry (chunk 0 enters at 0); the counts' exclusive sum, each
+//              chunk's first sample index;
+//   5. emit:   each chunk lists its starts; one thread per sample decodes it
+//              (the normals and sqrt(nu / chi2));
+//   6. cdf:    u_ij = t_cdf((L z_i)_j sqrt(nu / chi2_i), nu).
+// A sample longer than kTChunk draws (d above ~50) overflows the maps; then
+// one thread walks the starts with len(p) instead (same values, slower).
+constexpr int kTChunk = 64;
+constexpr uint32_t kLenBad = 0xffffu;
+constexpr uint8_t kMapOver = 0xff;  // exit offset >= kTChunk
+constexpr uint8_t kMapBad = 0xfe;   // ran past the decoded positions
+
+struct TMap {
+  uint8_t e[kTChunk];
+};
+// first a, then b (codes >= kTChunk stay)
+struct TCompose {
+  __host__ __device__ TMap operator()(const TMap& a, const TMap& b) const {
+    TMap r;
+#pragma unroll
+    for (int i = 0; i < kTChunk; ++i) r.e[i] = a.e[i] >= kTChunk ? a.e[i] : b.e[a.e[i]];
+    return r;
+  }
+};
+
+struct TParams {
+  double nu, d, c;  // d = nu/2 - 1/3, c = 1 / sqrt(9 d) (student_t.rs:104-105)
+  double lb;        // log B(nu/2, 1/2) by the reference's Lanczos log-gamma
+  uint32_t dim;
+};
+
+__global__ __launch_bounds__(kBulkBlock) void seq_u64_kernel(const StreamDesc sd,
+                                                             const uint64_t n,
+                                                             uint64_t* __restrict__ out) {
+  __shared__ uint64_t tile[kBulkBlock][kUniPer + 1];
+  const uint64_t chunk = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kBulkBlock * kUniPer;
+  Pcg g{stream_at(&sd, chunk)};
+#pragma unroll
+  for (int j = 0; j < kUniPer; ++j) tile[threadIdx.x][j] = g.next(sd.inc);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kUniPer; ++i) {
+    const int e = i * kBulkBlock + threadIdx.x;
+    const uint64_t idx = base + (uint64_t)e;
+    if (idx < n) out[idx] = tile[e / kUniPer][e % kUniPer];
+  }
+}
+
+// One normal from the draws at q (q moves past it); false if it needs draws
+// at or past n.
+__device__ __forceinline__ bool normal_from(const uint64_t* __restrict__ U, uint64_t n,
+                                            uint64_t& q, const ZigLds* z, double& x) {
+  ZigFeed<false> zf;
+  for (;;) {
+    if (q >= n) return false;
+    if (zf.feed(U[q++], z, x, true)) return true;
+  }
+}
+
+// The sample starting at p: its draw count (kLenBad when it runs past n);
+// with kKeep its normals into zo and sqrt(nu / chi2) into *scale.
+template <bool kKeep>
+__device__ uint32_t t_sample(const uint64_t* __restrict__ U, uint64_t n, uint64_t p,
+                             const TParams& tp, const ZigLds* z, double* __restrict__ zo,
+                             double* __restrict__ scale) {
+  uint64_t q = p;
+  double x, v;
+  for (;;) {  // student_t.rs:106-118
+    if (!normal_from(U, n, q, z, x)) return kLenBad;
+    const double vi = 1.0 + tp.c * x;
+    if (vi <= 0.0) continue;
+    v = vi * vi * vi;
+    if (q >= n) return kLenBad;
+    const double u = (double)(U[q++] >> 11) * 0x1p-53;  // pcg64.rs:37-39
+    if (u < 1.0 - 0.0331 * (x * x) * (x * x)) break;
+    if (log(u) < 0.5 * x * x + tp.d * (1.0 - v + log(v))) break;
+  }
+  if constexpr (kKeep) *scale = sqrt(tp.nu / (2.0 * (tp.d * v)));  // :156-157
+  for (uint32_t j = 0; j < tp.dim; ++j) {
+    double zz;
+    if (!normal_from(U, n, q, z, zz)) return kLenBad;
+    if constexpr (kKeep) zo[j] = zz;
+  }
+  const uint64_t len = q - p;
+  return len >= kLenBad ? kLenBad : (uint32_t)len;
+}
+
+__global__ __launch_bounds__(kBulkBlock) void t_len_kernel(const uint64_t* __restrict__ U,
+                                                           const uint64_t n_tot,
+                                                           const uint64_t n_pos,
+                                                           const TParams tp,
+                                                           uint16_t* __restrict__ len) {
+  __shared__ ZigLds zig;
+  zig_load(&zig);
+  __syncthreads();
+  const uint64_t p = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (p >= n_pos) return;
+  len[p] = (uint16_t)t_sample<false>(U, n_tot, p, tp, &zig, nullptr, nullptr);
+}
+
+// thread (chunk, entry offset): walk the starts to the chunk's end
+__global__ __launch_bounds__(kBulkBlock) void t_map_kernel(const uint16_t* __restrict__ len,
+                                                           const uint64_t n_chunks,
+                                                           TMap* __restrict__ maps,
+                                                           uint8_t* __restrict__ cnt) {
+  const uint64_t t = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  const uint64_t c = t / kTChunk;
+  const uint32_t e = (uint32_t)(t % kTChunk);
+  if (c >= n_chunks) return;
+  const uint64_t end = (c + 1) * kTChunk;
+  uint64_t p = c * kTChunk + e;
+  uint32_t k = 0;
+  uint8_t code = 0;
+  while (p < end) {
+    const uint32_t l = len[p];
+    if (l == kLenBad) {
+      code = kMapBad;
+      break;
+    }
+    p += l;
+    ++k;
+  }
+  if (code == 0) code = p - end < (uint64_t)kTChunk ? (uint8_t)(p - end) : kMapOver;
+  maps[c].e[e] = code;
+  cnt[c * kTChunk + e] = (uint8_t)k;
+}
+
+// true entry of every chunk from the composed maps; flag bit 0: a chunk's
+// entry overflowed, bit 1: ran past the decoded positions
+__global__ __launch_bounds__(kBulkBlock) void t_entry_kernel(const TMap* __restrict__ pre,
+                                                             const uint8_t* __restrict__ cnt,
+                                                             const uint64_t n_chunks,
+                                                             uint8_t* __restrict__ entry,
+                                                             uint64_t* __restrict__ cnt_true,
+                                                             unsigned* __restrict__ flag) {
+  const uint64_t c = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (c >= n_chunks) return;
+  const uint8_t e = c == 0 ? 0 : pre[c - 1].e[0];
+  entry[c] = e;
+  if (e < kTChunk) {
+    cnt_true[c] = cnt[c * kTChunk + e];
+  } else {
+    cnt_true[c] = 0;
+    atomicOr(flag, e == kMapOver ? 1u : 2u);
+  }
+}
+
+// each chunk lists its starts (sample index off[c] + k)
+__global__ __launch_bounds__(kBulkBlock) void t_starts_kernel(const uint16_t* __restrict__ len,
+                                                              const uint64_t n_chunks,
+                                                              const uint8_t* __restrict__ entry,
+                                                              const uint64_t* __restrict__ cnt_true,
+                                                              const uint64_t* __restrict__ off,
+                                                              const uint64_t n_samples,
+                                                              uint64_t* __restrict__ starts) {
+  const uint64_t c = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (c >= n_chunks || entry[c] >= kTChunk) return;
+  uint64_t p = c * kTChunk + entry[c];
+  const uint64_t o = off[c], k = cnt_true[c];
+  for (uint64_t j = 0; j < k && o + j < n_samples; ++j) {
+    starts[o + j] = p;
+    p += len[p];
+  }
+}
+
+// one thread walks the starts (maps overflowed: samples above kTChunk draws);
+// *found = the samples whose starts lie below n_pos
+__global__ void t_walk_kernel(const uint16_t* __restrict__ len, const uint64_t n_pos,
+                              const uint64_t n_samples, uint64_t* __restrict__ starts,
+                              uint64_t* __restrict__ found) {
+  uint64_t p = 0, i = 0;
+  for (; i < n_samples && p < n_pos; ++i) {
+    const uint32_t l = len[p];
+    if (l == kLenBad) break;
+    starts[i] = p;
+    p += l;
+  }
+  *found = i;
+}
+
+__global__ __launch_bounds__(kBulkBlock) void t_emit_kernel(const uint64_t* __restrict__ U,
+                                                            const uint64_t n_tot,
+                                                            const uint64_t* __restrict__ starts,
+                                                            const uint64_t n_samples,
+                                                            const TParams tp,
+                                                            double* __restrict__ Z,
+                                                            double* __restrict__ S) {
+  __shared__ ZigLds zig;
+  zig_load(&zig);
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (i >= n_samples) return;
+  (void)t_sample<true>(U, n_tot, starts[i], tp, &zig, Z + i * tp.dim, S + i);
+}
+
+// student_t.rs:19-63: I_x(a, b) with log B(a, b) = lb (the same for (b, a)):
+// 1 - I_{1-x}(b, a) past the mean, else the continued fraction (modified
+// Lentz, EPS 3e-7, at most 200 steps)
+__device__ double t_ibeta(double x, double a, double b, double lb) {
+  if (x <= 0.0) return 0.0;
+  if (x >= 1.0) return 1.0;
+  bool flip = false;
+  if (x > (a + 1.0) / (a + b + 2.0)) {
+    flip = true;
+    x = 1.0 - x;
+    const double t = a;
+    a = b;
+    b = t;
+  }
+  const double front = exp(a * log(x) + b * log(1.0 - x) - lb) / a;
+  const double eps = 3e-7, fpmin = 1e-30;
+  const double qab = a + b, qap = a + 1.0, qam = a - 1.0;
+  double c = 1.0;
+  double d = fmax(1.0 - qab * x / qap, fpmin);
+  d = 1.0 / d;
+  double h = d;
+  for (int m = 1; m <= 200; ++m) {
+    const double mf = (double)m, m2 = 2.0 * mf;
+    double aa = mf * (b - mf) * x / ((qam + m2) * (a + m2));
+    d = fmax(fabs(1.0 + aa * d), fpmin);
+    c = fmax(fabs(1.0 + aa / c), fpmin);
+    d = 1.0 / d;
+    h *= d * c;
+    aa = -(a + mf) * (qab + mf) * x / ((a + m2) * (qap + m2));
+    d = fmax(fabs(1.0 + aa * d), fpmin);
+    c = fmax(fabs(1.0 + aa / c), fpmin);
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < eps) break;
+  }
+  const double r = front * h;
+  return flip ? 1.0 - r : r;
+}
+
+// student_t.rs:8-16 and :160-165: x_j = sum_{k<=j} L[j][k] z_k from 0.0 in
+// k order, u = t_cdf(x_j scale, nu)
+__global__ __launch_bounds__(kBulkBlock) void t_cdf_kernel(const double* __restrict__ Z,
+                                                           const double* __restrict__ S,
+                                                           const double* __restrict__ L,
+                                                           const uint64_t n, const TParams tp,
+                                                           double* __restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t i = t / tp.dim;
+  const uint32_t j = (uint32_t)(t % tp.dim);
+  const double* zi = Z + i * tp.dim;
+  double xj = 0.0;
+  for (uint32_t k = 0; k <= j; ++k) xj += L[(uint64_t)j * tp.dim + k] * zi[k];
+  const double x = xj * S[i];
+  const double zz = tp.nu / (tp.nu + x * x);
+  const double ib = t_ibeta(zz, tp.nu / 2.0, 0.5, tp.lb);
+  out[t] = x >= 0.0 ? 1.0 - 0.5 * ib : 0.5 * ib;
+}
+
+// student_t.rs:67-91 on the host (the platform libm, as the reference's)
+double lgamma_lanczos_host(double x) {
+  static const double C[9] = {0.99999999999980993,  676.5203681218851,     -1259.1392167224028,
+                              771.32342877765313,   -176.61502916214059,   12.507343278686905,
+                              -0.13857109526572012, 9.9843695780195716e-6, 1.5056327351493116e-7};
+  if (x < 0.5) return log(M_PI) - log(sin(M_PI * x)) - lgamma_lanczos_host(1.0 - x);
+  x -= 1.0;
+  double acc = C[0];
+  for (int i = 1; i < 9; ++i) acc += C[i] / (x + (double)i);
+  const double t = x + 7.0 + 0.5;
+  return 0.5 * log(2.0 * M_PI) + (x + 0.5) * log(t) - t + log(acc);
+}
+
 }  // namespace
 }  // namespace stocha
 
@@ -472,6 +755,139 @@ int stocha_gaussian_copula(const double* corr, uint64_t rows, uint64_t cols, uin
   if (host) STOCHA_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   STOCHA_HIP(hipStreamSynchronize(st));
   return STOCHA_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// copula/student_t.rs:135-169 (student_t_copula_samples, binding
+// lib.rs:563-578): row i = t_cdf((L z_i) sqrt(nu / chi2_i), nu) on the stream
+// Pcg64Dxsm::new(seed); corr is (rows, cols) row-major on the host.
+int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, double nu,
+                            uint64_t n_samples, uint64_t seed, double* out, int where,
+                            char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  // lib.rs:570-572 (a NaN nu is refused too: the reference's gamma loop
+  // would never accept)
+  if (!(nu > 2.0)) return err.set(STOCHA_EINVAL, "nu must be > 2");
+  const uint64_t d = rows;
+  if (cols != d) return err.set(STOCHA_EINVAL, "correlation matrix must be square");
+  std::vector<double> L(d * d, 0.0);
+  if (d > 0)
+    if (int rc = cholesky_host(corr, d, L.data(), err)) return rc;  // gaussian.rs:75-95
+  if (!out) return STOCHA_OK;  // validate only
+  const uint64_t n = n_samples * d;
+  if (n == 0) return STOCHA_OK;
+  if (d > 0x7fffffffull) return err.set(STOCHA_EINVAL, "dimension too large");
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  hipStream_t st = globals().stream;
+  uint64_t state[2], inc[2];
+  stocha_rng_new(seed, 0, state, inc);
+  StreamDesc sd;
+  seq_desc(state, inc, kUniPer, &sd);
+  TParams tp;
+  tp.nu = nu;
+  tp.d = nu / 2.0 - 1.0 / 3.0;
+  tp.c = 1.0 / sqrt(9.0 * tp.d);
+  tp.lb = lgamma_lanczos_host(nu / 2.0) + lgamma_lanczos_host(0.5) -
+          lgamma_lanczos_host(nu / 2.0 + 0.5);  // student_t.rs:28
+  tp.dim = (uint32_t)d;
+  const bool host = where != STOCHA_DEVICE;
+  // draws per sample ~2.1 + 1.03 d; positions with margin, grown if short
+  uint64_t n_chunks = (uint64_t)((double)n_samples * (2.4 + 1.08 * (double)d)) / kTChunk + 4;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    const uint64_t n_pos = n_chunks * kTChunk;
+    const uint64_t n_tot = n_pos + 16 * (d + 8) + 1024;  // the last samples' draws
+    if (n_tot > (1ull << 40)) return err.set(STOCHA_EINVAL, "size too large");
+    const size_t nc = n_chunks;
+    size_t tmp_a = 0, tmp_b = 0;
+    STOCHA_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, tmp_a, (const TMap*)nullptr,
+                                                 (TMap*)nullptr, TCompose(), (int)nc, st));
+    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_b, (const uint64_t*)nullptr,
+                                                (uint64_t*)nullptr, (int)nc, st));
+    size_t tmp = tmp_a > tmp_b ? tmp_a : tmp_b;
+    const size_t sz[] = {8 * n_tot,     2 * n_pos,     sizeof(TMap) * nc, sizeof(TMap) * nc,
+                         kTChunk * nc,  nc,            8 * nc,            8 * nc,
+                         256,           tmp,           8 * n_samples,     8 * n,
+                         8 * n_samples, 8 * d * d,     host ? 8 * n : 0};
+    size_t o[16], total = 0;
+    for (int i = 0; i < 15; ++i) {
+      o[i] = total;
+      total += al256(sz[i]);
+    }
+    Workspace* ws = nullptr;
+    if (int rc = workspace(total, 0, &ws, err)) return rc;
+    char* b = static_cast<char*>(ws->dev);
+    uint64_t* U = reinterpret_cast<uint64_t*>(b + o[0]);
+    uint16_t* len = reinterpret_cast<uint16_t*>(b + o[1]);
+    TMap* maps = reinterpret_cast<TMap*>(b + o[2]);
+    TMap* pre = reinterpret_cast<TMap*>(b + o[3]);
+    uint8_t* cnt = reinterpret_cast<uint8_t*>(b + o[4]);
+    uint8_t* entry = reinterpret_cast<uint8_t*>(b + o[5]);
+    uint64_t* cnt_true = reinterpret_cast<uint64_t*>(b + o[6]);
+    uint64_t* off = reinterpret_cast<uint64_t*>(b + o[7]);
+    unsigned* flag = reinterpret_cast<unsigned*>(b + o[8]);
+    uint64_t* found = reinterpret_cast<uint64_t*>(b + o[8] + 64);
+    void* scan_tmp = b + o[9];
+    uint64_t* starts = reinterpret_cast<uint64_t*>(b + o[10]);
+    double* Z = reinterpret_cast<double*>(b + o[11]);
+    double* S = reinterpret_cast<double*>(b + o[12]);
+    double* dl = reinterpret_cast<double*>(b + o[13]);
+    double* dout = host ? reinterpret_cast<double*>(b + o[14]) : out;
+
+    const auto blocks = [](uint64_t items, uint64_t per) { return (unsigned)((items + per - 1) / per); };
+    KernelTimer timer(st);
+    timer.start();
+    seq_u64_kernel<<<blocks(n_tot, (uint64_t)kBulkBlock * kUniPer), kBulkBlock, 0, st>>>(sd, n_tot,
+                                                                                        U);
+    t_len_kernel<<<blocks(n_pos, kBulkBlock), kBulkBlock, 0, st>>>(U, n_tot, n_pos, tp, len);
+    t_map_kernel<<<blocks(n_pos, kBulkBlock), kBulkBlock, 0, st>>>(len, n_chunks, maps, cnt);
+    STOCHA_HIP(hipGetLastError());
+    STOCHA_HIP(hipcub::DeviceScan::InclusiveScan(scan_tmp, tmp, maps, pre, TCompose(), (int)nc,
+                                                 st));
+    STOCHA_HIP(hipMemsetAsync(flag, 0, 256, st));
+    t_entry_kernel<<<blocks(nc, kBulkBlock), kBulkBlock, 0, st>>>(pre, cnt, n_chunks, entry,
+                                                                  cnt_true, flag);
+    STOCHA_HIP(hipGetLastError());
+    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt_true, off, (int)nc, st));
+    unsigned h_flag = 0;
+    uint64_t last_off = 0, last_cnt = 0;
+    STOCHA_HIP(hipMemcpyAsync(&h_flag, flag, 4, hipMemcpyDeviceToHost, st));
+    STOCHA_HIP(hipMemcpyAsync(&last_off, off + nc - 1, 8, hipMemcpyDeviceToHost, st));
+    STOCHA_HIP(hipMemcpyAsync(&last_cnt, cnt_true + nc - 1, 8, hipMemcpyDeviceToHost, st));
+    STOCHA_HIP(hipStreamSynchronize(st));
+    if (h_flag & 1u) {  // a sample above kTChunk draws: walk the starts on one thread
+      t_walk_kernel<<<1, 1, 0, st>>>(len, n_pos, n_samples, starts, found);
+      STOCHA_HIP(hipGetLastError());
+      uint64_t h_found = 0;
+      STOCHA_HIP(hipMemcpyAsync(&h_found, found, 8, hipMemcpyDeviceToHost, st));
+      STOCHA_HIP(hipStreamSynchronize(st));
+      if (h_found < n_samples) {
+        n_chunks += n_chunks / 4 + 4;
+        continue;
+      }
+    } else {
+      if (last_off + last_cnt < n_samples) {  // too few positions: decode more
+        n_chunks += n_chunks / 4 + 4;
+        continue;
+      }
+      t_starts_kernel<<<blocks(nc, kBulkBlock), kBulkBlock, 0, st>>>(len, n_chunks, entry,
+                                                                     cnt_true, off, n_samples,
+                                                                     starts);
+    }
+    t_emit_kernel<<<blocks(n_samples, kBulkBlock), kBulkBlock, 0, st>>>(U, n_tot, starts,
+                                                                        n_samples, tp, Z, S);
+    STOCHA_HIP(hipMemcpyAsync(dl, L.data(), 8 * d * d, hipMemcpyHostToDevice, st));
+    t_cdf_kernel<<<blocks(n, kBulkBlock), kBulkBlock, 0, st>>>(Z, S, dl, n, tp, dout);
+    STOCHA_HIP(hipGetLastError());
+    timer.stop(4);
+    if (host) STOCHA_HIP(hipMemcpyAsync(out, dout, 8 * n, hipMemcpyDeviceToHost, st));
+    STOCHA_HIP(hipStreamSynchronize(st));
+    return STOCHA_OK;
+  }
+  return err.set(STOCHA_EDEVICE, "could not decode enough samples");
 }
 
 }  // extern "C"

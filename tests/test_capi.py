@@ -189,9 +189,34 @@ def test_greeks_pathwise_validation(bad, greeks, msg):
 
 
 def test_out_of_scope_functions_say_so():
-    for name in ("sobol", "student_t_copula", "sabr_implied_vol"):
+    for name in ("sobol", "halton", "sabr_implied_vol"):
         with pytest.raises(NotImplementedError, match="hot path"):
             getattr(S, name)()
+
+
+TC = dict(corr=np.array([[1.0, 0.5], [0.5, 1.0]]), nu=5.0, n_samples=10)
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (dict(nu=2.0), "nu must be > 2"),
+    (dict(nu=1.0), "nu must be > 2"),
+    (dict(nu=float("nan")), "nu must be > 2"),
+    (dict(corr=np.ones((2, 3))), "correlation matrix must be square"),
+    (dict(corr=np.array([[1.0, 1.5], [1.5, 1.0]])),
+     "matrix is not positive-definite (diagonal element 1 = -1.25)"),
+])
+def test_student_t_copula_validation(bad, msg):
+    # lib.rs:570-572 checks nu first, then student_t.rs:146-153 and the
+    # Cholesky of gaussian.rs:75-95; no diagonal or symmetry check here
+    with pytest.raises(ValueError, match=re.escape(msg)):
+        S.student_t_copula(**{**TC, **bad})
+
+
+def test_student_t_copula_takes_lists_like_the_wrapper():
+    # python/stocha/__init__.py:446-448 converts corr with np.asarray; an
+    # invalid nu is still reported before any device work
+    with pytest.raises(ValueError, match="nu must be > 2"):
+        S.student_t_copula([[1.0, 0.5], [0.5, 1.0]], nu=2.0, n_samples=3)
 
 
 def test_no_device_is_a_loud_error():

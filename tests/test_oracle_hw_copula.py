@@ -70,5 +70,12 @@ def test_hw_and_copula_validation():
         S.gaussian_copula(np.array([[1.0, 0.5], [0.4, 1.0]]), 10)
     with pytest.raises(ValueError, match="not positive-definite"):
         S.gaussian_copula(np.array([[1.0, 1.5], [1.5, 1.0]]), 10)
+    # the extension function takes float64 arrays only (PyReadonlyArray2<f64>,
+    # lib.rs:540); the package wrapper converts first (np.asarray, as
+    # python/stocha/__init__.py:421-423 does), so a float32 matrix gets as far
+    # as the checks
+    from stocha_amd import _stocha
     with pytest.raises(TypeError):
-        S.gaussian_copula(np.eye(2, dtype=np.float32), 10)
+        _stocha.gaussian_copula(np.eye(2, dtype=np.float32), 10)
+    with pytest.raises(ValueError, match="not positive-definite"):
+        S.gaussian_copula(np.array([[1.0, 1.5], [1.5, 1.0]], dtype=np.float32), 10)
