@@ -268,6 +268,22 @@ def main():
                             "hbm_gbs_written": nbulk * 8 / (kb * 1e-3) / 1e9}
         out["rng_class_100M"] = rng_ex
 
+        # single-stream copulas, 1M samples of dimension 4 (kernel time: the
+        # GPU work; wall time adds the 32 MB copy to pageable host memory)
+        cop = {}
+        ccorr = np.array([[1.0, 0.5, 0.2, 0.1], [0.5, 1.0, 0.3, 0.2],
+                          [0.2, 0.3, 1.0, 0.4], [0.1, 0.2, 0.4, 1.0]])
+        ncop = 1_000_000
+        for name, fn in (("gaussian", lambda: S.gaussian_copula(ccorr, n_samples=ncop, seed=SEED)),
+                         ("student_t_nu5", lambda: S.student_t_copula(ccorr, nu=5.0,
+                                                                      n_samples=ncop,
+                                                                      seed=SEED))):
+            tw = time_calls(fn, 2, 1)
+            kb = kernel_ms(S, fn, reps=2)
+            cop[name] = {"samples": ncop, "dim": 4, "wall_ms": tw * 1e3, "kernel_ms": kb,
+                         "samples_per_s_kernel": ncop / (kb * 1e-3)}
+        out["copula_1Mx4"] = cop
+
         # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
         from stocha_amd import batch as B
         inst = B.config4_instances(4096, seed=SEED)
