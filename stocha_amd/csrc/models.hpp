@@ -199,7 +199,7 @@ struct Heston32 {
   __device__ __forceinline__ void apply(float dw1, float dz) {
     float dw2 = fmaf(c.rho_f, dw1, c.rho_comp_f * dz);
     float v_plus = fmaxf(v, 0.f);
-    // v_sqrt_f32 (1 ulp); the FP32 form is judged against the FP64 price at 1e-4
+    // v_sqrt_f32 (1 ulp); the FP32 form's error bar is 1e-4 of the FP64 price
     float sqrt_v = __builtin_amdgcn_sqrtf(v_plus);
     // v += kappa*(theta - v+)*dt + xi*sqrt(v+)*sqrt(dt)*dw2
     v = fmaf(c.kappa_dt_f, c.theta_f - v_plus, fmaf(c.xi_sqdt_f * sqrt_v, dw2, v));
