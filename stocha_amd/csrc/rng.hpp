@@ -127,6 +127,24 @@ __device__ __forceinline__ U128 stream_at(const StreamDesc* __restrict__ d, uint
   return s;
 }
 
+// The affine map of the state over `paths` consecutive streams (block steps
+// each), composed from the jump table: a thread that walks paths i, i + paths,
+// ... moves its path's first state with one multiply-add per path instead of
+// stream_at's walk over the bits of the path index.  `paths` is uniform.
+__device__ __forceinline__ Jump stream_stride(const StreamDesc* __restrict__ d, uint64_t paths) {
+  Jump r{U128{1, 0}, U128{0, 0}};
+#pragma unroll 1
+  for (int k = 0; k < kJumpBits; ++k) {
+    if ((paths >> k) == 0) break;
+    if ((paths >> k) & 1ull) {
+      const Jump j = d->jump[k];
+      r.a = mul128(j.a, r.a);
+      r.c = add128(mul128(j.a, r.c), j.c);
+    }
+  }
+  return r;
+}
+
 // u32 -> f32 of the high word of a draw.  LLVM folds uitofp(trunc(x >> 32))
 // back into a 64-bit conversion (six instructions on gfx950); the empty asm
 // makes the 32-bit operand opaque so a single v_cvt_f32_u32 is emitted.  (No
