@@ -81,12 +81,17 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchInstDev* __res
     const StreamDesc& sd = descs[in.desc];
     double s1 = 0.0, s2 = 0.0;
     const U128 inc = sd.inc;
-    // every lane runs a path (lanes past the tile repeat its first path) so
-    // that the queue schedule sees whole wavefronts; only valid lanes count
+    // every lane runs a path so that the queue schedule sees whole
+    // wavefronts; only valid lanes count
+    // path k + kBlock's first state is k's moved by one affine map (lanes
+    // past the tile draw from their own unused streams)
+    const Jump next = stream_stride(&sd, kBlock);
+    U128 first = stream_at(&sd, tl.path0 + threadIdx.x);
     for (uint32_t j = 0; j * kBlock < tl.count; ++j) {
       const uint32_t k = j * kBlock + threadIdx.x;
       const bool valid = k < tl.count;
-      Pcg g{stream_at(&sd, tl.path0 + (valid ? k : 0))};
+      Pcg g{first};
+      first = add128(mul128(next.a, first), next.c);
       double st;
       if (in.model == kGbm)
         st = queue_path<GbmSrc<true>, Gbm32, 16, kBlock>(in.g, g, inc, in.steps, &zig,
