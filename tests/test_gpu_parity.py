@@ -192,6 +192,41 @@ def test_lsmc_paths_per_thread_match_oracle(gpu, orc, n_paths):
     assert se == pytest.approx(rse, rel=1e-9)
 
 
+@pytest.mark.parametrize("n_paths", [262_144, 262_145, 524_289, 1_048_577, 4_194_304])
+def test_lsmc_paths_per_thread_boundaries(gpu, orc, n_paths):
+    # on 256 CUs: 1 path per thread exactly; one path over (2 per thread, the
+    # second half partly empty); just over 2 and 4 per thread; 16 per thread
+    # with every slot used -- the per-path checks of the second half of a
+    # thread's paths and the exact fit
+    kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.2, t=1.0, steps=4, n_paths=n_paths,
+              is_put=True, poly_degree=3, seed=13)
+    p, se = gpu.lsmc_american_option(**kw)
+    rp, rse = orc.lsmc(**kw)
+    assert p == pytest.approx(rp, rel=1e-9)
+    assert se == pytest.approx(rse, rel=1e-9)
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3])
+def test_lsmc_fewest_dates(gpu, orc, steps):
+    # one step: no regression date (the per-date kernels); two: one date in
+    # the cooperative induction
+    kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.2, t=1.0, steps=steps, n_paths=70_001,
+              is_put=False, poly_degree=2, seed=5)
+    p, se = gpu.lsmc_american_option(**kw)
+    rp, rse = orc.lsmc(**kw)
+    assert p == pytest.approx(rp, rel=1e-9)
+    assert se == pytest.approx(rse, rel=1e-9)
+
+
+@pytest.mark.parametrize("n_paths", [262_145, 4_194_304])
+def test_lsmc_fp32_boundaries_agree_with_fp64(gpu, n_paths):
+    kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.2, t=1.0, steps=4, n_paths=n_paths,
+              is_put=True, poly_degree=3, seed=13)
+    p32, se32 = gpu.lsmc_american_option_ex(**kw, precision=32)
+    p64, se64 = gpu.lsmc_american_option(**kw)
+    assert abs(p32 - p64) < 0.05 * se64, (p32, p64, se64)
+
+
 def test_lsmc_fp32_close_to_oracle(gpu, orc):
     kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.2, t=1.0, steps=50, n_paths=50_000,
               is_put=True, poly_degree=3, seed=42)
