@@ -350,9 +350,11 @@ __global__ __launch_bounds__(kBulkBlock) void gauss_copula_kernel(const double* 
 //   3. maps:   positions in chunks of kTChunk; per chunk and entry offset e
 //              (the first start in the chunk is at chunk + e) the exit
 //              offset into the next chunk and the number of starts;
-//   4. scan:   an inclusive scan composing the maps gives every chunk's true
-//              entry (chunk 0 enters at 0); the counts' exclusive sum, each
-//              chunk's first sample index;
+//   4. entries: each block of 256 chunks composes their maps (a tree); one
+//              thread applies the blocks' compositions in order to entry 0
+//              (chunk 0 enters at 0), giving each block's entry; each block
+//              then walks its own chunks' maps from it (maps in LDS).  The
+//              counts' exclusive sum gives each chunk's first sample index;
 //   5. emit:   each chunk lists its starts; one thread per sample decodes it
 //              (the normals and sqrt(nu / chi2));
 //   6. cdf:    u_ij = t_cdf((L z_i)_j sqrt(nu / chi2_i), nu).
@@ -363,18 +365,49 @@ constexpr uint32_t kLenBad = 0xffffu;
 constexpr uint8_t kMapOver = 0xff;  // exit offset >= kTChunk
 constexpr uint8_t kMapBad = 0xfe;   // ran past the decoded positions
 
-struct TMap {
+struct __align__(16) TMap {
   uint8_t e[kTChunk];
 };
-// first a, then b (codes >= kTChunk stay)
-struct TCompose {
-  __host__ __device__ TMap operator()(const TMap& a, const TMap& b) const {
-    TMap r;
+// r = first a, then b (codes >= kTChunk stay), on maps in LDS (16-byte
+// aligned rows): a and r move as four 16-byte words, b is read per entry
+__device__ __forceinline__ void t_compose(const uint8_t* __restrict__ a,
+                                          const uint8_t* __restrict__ b, uint8_t* __restrict__ r) {
 #pragma unroll
-    for (int i = 0; i < kTChunk; ++i) r.e[i] = a.e[i] >= kTChunk ? a.e[i] : b.e[a.e[i]];
-    return r;
+  for (int w = 0; w < kTChunk / 16; ++w) {
+    const uint4 av = reinterpret_cast<const uint4*>(a)[w];
+    const uint32_t in[4] = {av.x, av.y, av.z, av.w};
+    uint32_t out[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t o = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t x = (in[q] >> (8 * k)) & 0xffu;
+        const uint32_t y = x >= (uint32_t)kTChunk ? x : (uint32_t)b[x];
+        o |= y << (8 * k);
+      }
+      out[q] = o;
+    }
+    reinterpret_cast<uint4*>(r)[w] = make_uint4(out[0], out[1], out[2], out[3]);
   }
-};
+}
+
+__device__ __forceinline__ void t_copy_row(const uint8_t* __restrict__ a, uint8_t* __restrict__ r) {
+#pragma unroll
+  for (int w = 0; w < kTChunk / 16; ++w)
+    reinterpret_cast<uint4*>(r)[w] = reinterpret_cast<const uint4*>(a)[w];
+}
+
+// row i = the identity map (entry e -> e)
+__device__ __forceinline__ void t_ident_row(uint8_t* __restrict__ r) {
+#pragma unroll
+  for (int w = 0; w < kTChunk / 16; ++w) {
+    const uint32_t b0 = 16u * w;
+    const uint32_t x = (b0) | (b0 + 1) << 8 | (b0 + 2) << 16 | (b0 + 3) << 24;
+    reinterpret_cast<uint4*>(r)[w] = make_uint4(x, x + 0x04040404u, x + 0x08080808u,
+                                                x + 0x0c0c0c0cu);
+  }
+}
 
 struct TParams {
   double nu, d, c;  // d = nu/2 - 1/3, c = 1 / sqrt(9 d) (student_t.rs:104-105)
@@ -479,17 +512,77 @@ __global__ __launch_bounds__(kBulkBlock) void t_map_kernel(const uint16_t* __res
   cnt[c * kTChunk + e] = (uint8_t)k;
 }
 
-// true entry of every chunk from the composed maps; flag bit 0: a chunk's
-// entry overflowed, bit 1: ran past the decoded positions
-__global__ __launch_bounds__(kBulkBlock) void t_entry_kernel(const TMap* __restrict__ pre,
+// Level 1: each block composes its kBulkBlock chunks' maps in order (a tree
+// of pairwise compositions in LDS) into agg[block].
+__global__ __launch_bounds__(kBulkBlock) void t_agg_kernel(const TMap* __restrict__ maps,
+                                                           const uint64_t n_chunks,
+                                                           TMap* __restrict__ agg) {
+  __shared__ __align__(16) uint8_t m[2][kBulkBlock][kTChunk];
+  const uint64_t c = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (c < n_chunks)
+    t_copy_row(maps[c].e, m[0][threadIdx.x]);
+  else
+    t_ident_row(m[0][threadIdx.x]);  // identity past the last chunk
+  int cur = 0;
+  for (int w = kBulkBlock / 2; w >= 1; w >>= 1) {  // row t = rows 2t, 2t+1 composed
+    __syncthreads();
+    if ((int)threadIdx.x < w)
+      t_compose(m[cur][2 * threadIdx.x], m[cur][2 * threadIdx.x + 1], m[cur ^ 1][threadIdx.x]);
+    cur ^= 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) t_copy_row(m[cur][0], agg[blockIdx.x].e);
+}
+
+// Level 2 (one block): xin[b] = the entry of block b's first chunk, the
+// blocks' totals applied in order to entry 0 (chunk 0 enters at 0); tiles of
+// kBulkBlock totals staged in LDS, walked by one thread.
+__global__ __launch_bounds__(kBulkBlock) void t_xin_kernel(const TMap* __restrict__ agg,
+                                                           const uint64_t n_blocks,
+                                                           uint8_t* __restrict__ xin) {
+  __shared__ __align__(16) uint8_t m[kBulkBlock][kTChunk];
+  uint32_t x = 0;
+  for (uint64_t base = 0; base < n_blocks; base += kBulkBlock) {
+    const uint64_t b = base + threadIdx.x;
+    __syncthreads();
+    if (b < n_blocks) t_copy_row(agg[b].e, m[threadIdx.x]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t k = n_blocks - base < (uint64_t)kBulkBlock ? n_blocks - base : kBulkBlock;
+      for (uint64_t j = 0; j < k; ++j) {
+        xin[base + j] = (uint8_t)x;
+        x = x < (uint32_t)kTChunk ? m[j][x] : x;
+      }
+    }
+  }
+}
+
+// Level 3: each block walks its chunks from its entry: the true entry of
+// every chunk (thread 0, maps in LDS), then its count of starts; flag bit 0:
+// a chunk's entry overflowed, bit 1: ran past the decoded positions
+__global__ __launch_bounds__(kBulkBlock) void t_entry_kernel(const TMap* __restrict__ maps,
+                                                             const uint8_t* __restrict__ xin,
                                                              const uint8_t* __restrict__ cnt,
                                                              const uint64_t n_chunks,
                                                              uint8_t* __restrict__ entry,
                                                              uint64_t* __restrict__ cnt_true,
                                                              unsigned* __restrict__ flag) {
-  const uint64_t c = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  __shared__ __align__(16) uint8_t m[kBulkBlock][kTChunk];
+  __shared__ uint8_t ent[kBulkBlock];
+  const uint64_t c0 = (uint64_t)blockIdx.x * kBulkBlock, c = c0 + threadIdx.x;
+  if (c < n_chunks) t_copy_row(maps[c].e, m[threadIdx.x]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t k = n_chunks - c0 < (uint64_t)kBulkBlock ? n_chunks - c0 : kBulkBlock;
+    uint32_t x = xin[blockIdx.x];
+    for (uint64_t j = 0; j < k; ++j) {
+      ent[j] = (uint8_t)x;
+      x = x < (uint32_t)kTChunk ? m[j][x] : x;
+    }
+  }
+  __syncthreads();
   if (c >= n_chunks) return;
-  const uint8_t e = c == 0 ? 0 : pre[c - 1].e[0];
+  const uint8_t e = ent[threadIdx.x];
   entry[c] = e;
   if (e < kTChunk) {
     cnt_true[c] = cnt[c * kTChunk + e];
@@ -802,18 +895,17 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     const uint64_t n_tot = n_pos + 16 * (d + 8) + 1024;  // the last samples' draws
     if (n_tot > (1ull << 40)) return err.set(STOCHA_EINVAL, "size too large");
     const size_t nc = n_chunks;
-    size_t tmp_a = 0, tmp_b = 0;
-    STOCHA_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, tmp_a, (const TMap*)nullptr,
-                                                 (TMap*)nullptr, TCompose(), (int)nc, st));
-    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_b, (const uint64_t*)nullptr,
+    const uint64_t n_blk = (nc + kBulkBlock - 1) / kBulkBlock;
+    size_t tmp = 0;
+    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const uint64_t*)nullptr,
                                                 (uint64_t*)nullptr, (int)nc, st));
-    size_t tmp = tmp_a > tmp_b ? tmp_a : tmp_b;
-    const size_t sz[] = {8 * n_tot,     2 * n_pos,     sizeof(TMap) * nc, sizeof(TMap) * nc,
+    const size_t sz[] = {8 * n_tot,     2 * n_pos,     sizeof(TMap) * nc, 0,
                          kTChunk * nc,  nc,            8 * nc,            8 * nc,
                          256,           tmp,           8 * n_samples,     8 * n,
-                         8 * n_samples, 8 * d * d,     host ? 8 * n : 0};
-    size_t o[16], total = 0;
-    for (int i = 0; i < 15; ++i) {
+                         8 * n_samples, 8 * d * d,     host ? 8 * n : 0,
+                         sizeof(TMap) * n_blk, n_blk};
+    size_t o[17], total = 0;
+    for (int i = 0; i < 17; ++i) {
       o[i] = total;
       total += al256(sz[i]);
     }
@@ -823,7 +915,6 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     uint64_t* U = reinterpret_cast<uint64_t*>(b + o[0]);
     uint16_t* len = reinterpret_cast<uint16_t*>(b + o[1]);
     TMap* maps = reinterpret_cast<TMap*>(b + o[2]);
-    TMap* pre = reinterpret_cast<TMap*>(b + o[3]);
     uint8_t* cnt = reinterpret_cast<uint8_t*>(b + o[4]);
     uint8_t* entry = reinterpret_cast<uint8_t*>(b + o[5]);
     uint64_t* cnt_true = reinterpret_cast<uint64_t*>(b + o[6]);
@@ -836,6 +927,8 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     double* S = reinterpret_cast<double*>(b + o[12]);
     double* dl = reinterpret_cast<double*>(b + o[13]);
     double* dout = host ? reinterpret_cast<double*>(b + o[14]) : out;
+    TMap* agg = reinterpret_cast<TMap*>(b + o[15]);
+    uint8_t* xin = reinterpret_cast<uint8_t*>(b + o[16]);
 
     const auto blocks = [](uint64_t items, uint64_t per) { return (unsigned)((items + per - 1) / per); };
     KernelTimer timer(st);
@@ -845,11 +938,11 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     t_len_kernel<<<blocks(n_pos, kBulkBlock), kBulkBlock, 0, st>>>(U, n_tot, n_pos, tp, len);
     t_map_kernel<<<blocks(n_pos, kBulkBlock), kBulkBlock, 0, st>>>(len, n_chunks, maps, cnt);
     STOCHA_HIP(hipGetLastError());
-    STOCHA_HIP(hipcub::DeviceScan::InclusiveScan(scan_tmp, tmp, maps, pre, TCompose(), (int)nc,
-                                                 st));
+    t_agg_kernel<<<(unsigned)n_blk, kBulkBlock, 0, st>>>(maps, n_chunks, agg);
+    t_xin_kernel<<<1, kBulkBlock, 0, st>>>(agg, n_blk, xin);
     STOCHA_HIP(hipMemsetAsync(flag, 0, 256, st));
-    t_entry_kernel<<<blocks(nc, kBulkBlock), kBulkBlock, 0, st>>>(pre, cnt, n_chunks, entry,
-                                                                  cnt_true, flag);
+    t_entry_kernel<<<(unsigned)n_blk, kBulkBlock, 0, st>>>(maps, xin, cnt, n_chunks, entry,
+                                                            cnt_true, flag);
     STOCHA_HIP(hipGetLastError());
     STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt_true, off, (int)nc, st));
     unsigned h_flag = 0;
