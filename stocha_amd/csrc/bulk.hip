@@ -472,6 +472,43 @@ __device__ uint32_t t_sample(const uint64_t* __restrict__ U, uint64_t n, uint64_
   return len >= kLenBad ? kLenBad : (uint32_t)len;
 }
 
+// t_sample's draw count as one flat loop, one draw per iteration, so the
+// lanes of a wave stay in step (nested data-dependent loops had them in
+// different places of their samples).  phase 0: a gamma normal, 1: the gamma
+// uniform, 2 + j: the sample's normal j.
+__device__ uint32_t t_sample_len(const uint64_t* __restrict__ U, uint64_t n, uint64_t p,
+                                 const TParams& tp, const ZigLds* z) {
+  ZigFeed<false> zf;
+  const uint32_t end = 2u + tp.dim;
+  uint32_t phase = 0;
+  double x = 0.0;  // the gamma normal under test
+  uint64_t q = p;
+  while (phase < end) {
+    if (q >= n) return kLenBad;
+    const uint64_t u = U[q++];
+    if (phase == 1u) {  // student_t.rs:110-117
+      const double uf = (double)(u >> 11) * 0x1p-53;
+      const double vi = 1.0 + tp.c * x;
+      const double v = vi * vi * vi;
+      bool ok = uf < 1.0 - 0.0331 * (x * x) * (x * x);
+      if (!ok) ok = log(uf) < 0.5 * x * x + tp.d * (1.0 - v + log(v));
+      phase = ok ? 2u : 0u;
+    } else {
+      double y;
+      if (zf.feed(u, z, y, true)) {
+        if (phase == 0u) {
+          x = y;
+          phase = (1.0 + tp.c * y <= 0.0) ? 0u : 1u;  // :108-109
+        } else {
+          ++phase;
+        }
+      }
+    }
+  }
+  const uint64_t l = q - p;
+  return l >= kLenBad ? kLenBad : (uint32_t)l;
+}
+
 __global__ __launch_bounds__(kBulkBlock) void t_len_kernel(const uint64_t* __restrict__ U,
                                                            const uint64_t n_tot,
                                                            const uint64_t n_pos,
@@ -482,7 +519,7 @@ __global__ __launch_bounds__(kBulkBlock) void t_len_kernel(const uint64_t* __res
   __syncthreads();
   const uint64_t p = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
   if (p >= n_pos) return;
-  len[p] = (uint16_t)t_sample<false>(U, n_tot, p, tp, &zig, nullptr, nullptr);
+  len[p] = (uint16_t)t_sample_len(U, n_tot, p, tp, &zig);
 }
 
 // thread (chunk, entry offset): walk the starts to the chunk's end
