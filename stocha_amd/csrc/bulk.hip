@@ -99,10 +99,18 @@ __global__ __launch_bounds__(kBulkBlock) void norm_spec_kernel(const StreamDesc 
   Pcg g{stream_at(&sd, c)};
   uint64_t p = s, m = 0;
   uint32_t k = 0;
-  while (p < end) {
-    ++k;
-    m |= 1ull << (p - s);
-    (void)decode_one(g, sd.inc, &zig, p);
+  // one draw per iteration (the lanes of a wave stay in step; a per-normal
+  // loop would make every lane wait for the slowest lane's normal)
+  ZigFeed<false> zf;
+  bool start = true;  // a normal starts at p
+  while (!(start && p >= end)) {
+    if (start) {
+      ++k;
+      m |= 1ull << (p - s);
+    }
+    double x;
+    start = zf.feed(g.next(sd.inc), &zig, x, true);
+    ++p;
   }
   cnt[c] = k;
   mask[c] = m;
@@ -174,11 +182,17 @@ __global__ __launch_bounds__(kBulkBlock) void norm_emit_kernel(
   if (k == 0 || o >= n) return;
   const uint64_t e = c == 0 ? 0 : prev_exit[c - 1];
   Pcg g = state_at(sd, c, e);
-  uint64_t p = e;
-  for (uint64_t j = 0; j < k && o + j < n; ++j) {
-    const double z = decode_one(g, sd.inc, &zig, p);
-    out[o + j] = scaled ? loc + scale * z : z;  // sample_scaled (normal.rs:77-79)
-    if (o + j == n - 1) *consumed = p;
+  uint64_t p = e, j = 0;
+  ZigFeed<false> zf;  // one draw per iteration, as in the speculative pass
+  while (j < k && o + j < n) {
+    double z;
+    const bool done = zf.feed(g.next(sd.inc), &zig, z, true);
+    ++p;
+    if (done) {
+      out[o + j] = scaled ? loc + scale * z : z;  // sample_scaled (normal.rs:77-79)
+      if (o + j == n - 1) *consumed = p;
+      ++j;
+    }
   }
 }
 
