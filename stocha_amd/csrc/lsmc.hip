@@ -761,10 +761,10 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   double ex_mean = 0.0, ex_std = 1.0, ex_inv = 1.0, ex_c[kMaxCols];
   for (int j = 0; j < kMaxCols; ++j) ex_c[j] = 0.0;
   double c = a.k, sc = a.k;  // moment variable u = (S - c) / sc
-  // Up to 4 paths per thread, and with kRegSpot, the spots of the next date
-  // are loaded at the end of a pass, so their latency overlaps the
-  // reduction, barrier and solve.
-  constexpr bool kPrefetch = kPpt <= 4 || kRegSpot;
+  // Up to 4 paths per thread, with kRegSpot, and at 16 FP64 paths per thread
+  // (registers to spare there) the spots of the next date are loaded a pass
+  // ahead, so their latency overlaps the solve and the exercise pass.
+  constexpr bool kPrefetch = kPpt <= 4 || kRegSpot || kPpt == 16;
   SpotT nxt[kPrefetch ? kPpt : 1];
   if constexpr (kPrefetch) {
     const SpotCol<kFp32> first(a.paths, a.n, a.steps - 1);  // steps >= 2
