@@ -116,22 +116,28 @@ struct Heston64 {
 // Heston QE, log-price part (heston.rs:152-157); the variance recursion runs
 // in the draw source (HestonQeSrc), which hands over V(t+dt) with the
 // step's asset normal.  v tracks the same recursion (v0, then each V(t+dt)).
+// The state is ln S: the reference takes S = exp(ln S) every step and ln S
+// back from it on the next, and the pair is dropped here (S is formed only
+// when read).  log(exp(x)) returns x itself wherever |x| is above about 1,
+// since the rounding of exp then moves the logarithm by less than half an
+// ulp of x; nearer 0 it can differ by that 1e-16 relative, so a path differs
+// from the reference's by at most steps x 1e-16 relative.
 struct HestonQe64 {
   static constexpr bool kFp32 = false;
   HestonQeConst c;
-  double s, v;
-  __device__ __forceinline__ HestonQe64(const HestonQeConst& k, bool) : c(k), s(k.s0), v(k.v0) {}
+  double ls, v;
+  __device__ __forceinline__ HestonQe64(const HestonQeConst& k, bool)
+      : c(k), ls(log(k.s0)), v(k.v0) {}
   __device__ __forceinline__ void apply_qe(double v_next, double z1) {
     const double v_plus = fmax(v, 0.0);
     double var_term = c.k3s * v_plus + c.k4s * v_next;
     var_term = fmax(var_term, 0.0);
-    double log_s = log(s) + c.k0s + c.k1s * v_plus + c.k2s * v_next + sqrt(var_term) * z1;
+    double log_s = ls + c.k0s + c.k1s * v_plus + c.k2s * v_next + sqrt(var_term) * z1;
     // f64::clamp(-500, 500) keeps a NaN
-    log_s = log_s < -500.0 ? -500.0 : (log_s > 500.0 ? 500.0 : log_s);
-    s = exp(log_s);
+    ls = log_s < -500.0 ? -500.0 : (log_s > 500.0 ? 500.0 : log_s);
     v = v_next;
   }
-  __device__ __forceinline__ double terminal() const { return s; }
+  __device__ __forceinline__ double terminal() const { return exp(ls); }
 };
 
 // hull_white.rs:47-49: r(t+dt) = r(t) e + mean_reversion + vol z
