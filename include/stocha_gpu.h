@@ -31,7 +31,10 @@
  *     STOCHA_EINVAL).  Nothing is thrown across the ABI.
  *   - Outputs are caller-allocated.  `where` says whether `out` is host
  *     memory (STOCHA_HOST; the call returns after the copy-back) or device
- *     memory on the current HIP device (STOCHA_DEVICE).  For the
+ *     memory on the current HIP device (STOCHA_DEVICE).  Any other `where`,
+ *     an output whose byte count overflows 64 bits, and a STOCHA_DEVICE
+ *     output that is not device memory of the current device with room for
+ *     the result are refused with STOCHA_EINVAL before any launch.  For the
  *     path-returning functions out == NULL validates the arguments only
  *     (no device work), so a binding can raise the reference's error before
  *     it allocates the output.

@@ -376,6 +376,8 @@ __global__ __launch_bounds__(kBulkBlock) void gauss_copula_kernel(const double* 
 // one thread walks the starts with len(p) instead (same values, slower).
 constexpr int kTChunk = 64;
 constexpr uint32_t kLenBad = 0xffffu;
+// largest dimension: a sample takes ~2.1 + 1.03 d draws, far below kLenBad
+constexpr uint64_t kTMaxDim = 32768;
 constexpr uint8_t kMapOver = 0xff;  // exit offset >= kTChunk
 constexpr uint8_t kMapBad = 0xfe;   // ran past the decoded positions
 
@@ -790,6 +792,8 @@ int stocha_rng_new(uint64_t seed_lo, uint64_t seed_hi, uint64_t* state, uint64_t
 int stocha_rng_uniform(const uint64_t* state, const uint64_t* inc, uint64_t n, double* out,
                        int where, uint64_t* new_state, char* errbuf, size_t errlen) {
   Err err{errbuf, errlen};
+  size_t bytes = 0;
+  if (int rc = check_output(out, where, n, 1, sizeof(double), "out", &bytes, err)) return rc;
   advance_state(state, inc, n, new_state);
   if (n == 0) return STOCHA_OK;
   std::lock_guard<std::mutex> lk(globals().mu);
@@ -821,6 +825,8 @@ int stocha_rng_normal(const uint64_t* state, const uint64_t* inc, uint64_t n, do
                       uint64_t* consumed, char* errbuf, size_t errlen) {
   Err err{errbuf, errlen};
   if (scaled && scale <= 0.0) return err.set(STOCHA_EINVAL, "scale must be positive");
+  size_t bytes = 0;
+  if (int rc = check_output(out, where, n, 1, sizeof(double), "out", &bytes, err)) return rc;
   if (n == 0) {
     new_state[0] = state[0];
     new_state[1] = state[1];
@@ -871,6 +877,9 @@ int stocha_gaussian_copula(const double* corr, uint64_t rows, uint64_t cols, uin
   if (d > 0)
     if (int rc = cholesky_host(corr, d, L.data(), err)) return rc;
   if (!out) return STOCHA_OK;  // validate only
+  size_t bytes = 0;
+  if (int rc = check_output(out, where, n_samples, d, sizeof(double), "out", &bytes, err))
+    return rc;
   const uint64_t n = n_samples * d;
   if (n == 0) return STOCHA_OK;
   if (d > 0xffffffffull) return err.set(STOCHA_EINVAL, "dimension too large");
@@ -921,9 +930,17 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
   if (d > 0)
     if (int rc = cholesky_host(corr, d, L.data(), err)) return rc;  // gaussian.rs:75-95
   if (!out) return STOCHA_OK;  // validate only
+  size_t bytes = 0;
+  if (int rc = check_output(out, where, n_samples, d, sizeof(double), "out", &bytes, err))
+    return rc;
   const uint64_t n = n_samples * d;
   if (n == 0) return STOCHA_OK;
-  if (d > 0x7fffffffull) return err.set(STOCHA_EINVAL, "dimension too large");
+  // per-position sample lengths are 16-bit (0xffff marks a position whose
+  // sample runs past the decoded draws): ~1.03 d + 2 draws per sample fit
+  // with a wide margin up to this dimension
+  if (d > kTMaxDim)
+    return err.set(STOCHA_EINVAL, "the Student-t copula supports dimensions up to %llu, got %llu",
+                   (unsigned long long)kTMaxDim, (unsigned long long)d);
   std::lock_guard<std::mutex> lk(globals().mu);
   if (int rc = check_device(err)) return rc;
   hipStream_t st = globals().stream;

@@ -183,11 +183,14 @@ namespace {
 // Runs a path-storing launch into host or device memory.
 int store_paths(int model, const void* mc, uint64_t seed, uint64_t block, uint64_t n_rows,
                 uint64_t base_rows, uint64_t steps, double* out, int where, const Err& err) {
+  size_t bytes = 0;
+  if (steps == UINT64_MAX) return err.set(STOCHA_EINVAL, "steps is too large");
+  if (int rc = check_output(out, where, n_rows, steps + 1, sizeof(double), "out", &bytes, err))
+    return rc;
   if (int rc = check_device(err)) return rc;
   hipStream_t st = globals().stream;
   StreamDesc sd;
   make_stream_desc(seed, block, &sd);
-  const size_t bytes = sizeof(double) * n_rows * (steps + 1);
   if (where == STOCHA_DEVICE) {
     if (int rc = launch_paths(model, mc, sd, n_rows, base_rows, steps, out, st, err)) return rc;
     STOCHA_HIP(hipStreamSynchronize(st));
@@ -436,6 +439,13 @@ int stocha_multi_gbm(const double* s0, uint64_t n_s0, const double* mu, uint64_t
     return cholesky_host(corr, n, l.data(), err);
   }
   std::lock_guard<std::mutex> lk(globals().mu);
+  size_t row_vals = 0, bytes = 0;
+  if (steps == UINT64_MAX || !byte_count(steps + 1, n, 1, &row_vals))
+    return err.set(STOCHA_EINVAL, "out of %llu x %llu x %llu values does not fit in memory",
+                   (unsigned long long)n_paths, (unsigned long long)steps,
+                   (unsigned long long)n);
+  if (int rc = check_output(out, where, n_paths, row_vals, sizeof(double), "out", &bytes, err))
+    return rc;
   if (int rc = check_device(err)) return rc;
   return run_multi_gbm_paths(s0, mu, sigma, corr, n, t, steps, n_paths, seed, antithetic, out,
                              where, globals().stream, err);
@@ -676,6 +686,9 @@ int stocha_terminal_prices(int model, const double* params, uint64_t n_params, c
   if (int rc = fd_kind(model, scheme, &kind, err)) return rc;
   if (!out || n_paths == 0) return STOCHA_OK;
   std::lock_guard<std::mutex> lk(globals().mu);
+  size_t bytes = 0;
+  if (int rc = check_output(out, where, n_paths, 1, sizeof(double), "out", &bytes, err))
+    return rc;
   if (int rc = check_device(err)) return rc;
   const FdConsts c = fd_consts(kind, params, n_steps);
   const void* cp = c.ptr(kind);

@@ -95,6 +95,65 @@ inline int check_device(const Err& err) {
   return STOCHA_OK;
 }
 
+// ------------------------------------------------------- output checks --
+// n * m * elem, false when the product does not fit in 64 bits.
+inline bool byte_count(uint64_t n, uint64_t m, uint64_t elem, size_t* bytes) {
+  uint64_t nm = 0, b = 0;
+  if (__builtin_mul_overflow(n, m, &nm) || __builtin_mul_overflow(nm, elem, &b)) return false;
+  *bytes = (size_t)b;
+  return true;
+}
+
+// `bytes` bytes from p on must lie in one device allocation of the current
+// device: the kernels write device outputs directly, and a host pointer, a
+// pointer on another device or a short buffer would fault the GPU or write
+// past the caller's memory.
+inline int check_device_span(const void* p, size_t bytes, const char* name, const Err& err) {
+  if (!p) return err.set(STOCHA_EINVAL, "%s must not be NULL", name);
+  int dev = 0;
+  STOCHA_HIP(hipGetDevice(&dev));
+  hipPointerAttribute_t a{};
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // keep the failed query out of the next launch check
+    return err.set(STOCHA_EINVAL, "%s is not device memory of the current device", name);
+  }
+  if (a.type != hipMemoryTypeDevice || a.device != dev)
+    return err.set(STOCHA_EINVAL,
+                   "%s is not device memory of the current device (memory type %d, device %d; "
+                   "current device %d)", name, (int)a.type, a.device, dev);
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return err.set(STOCHA_EINVAL, "%s: the allocation it points into is unknown", name);
+  }
+  const uintptr_t lo = (uintptr_t)base, q = (uintptr_t)p;
+  if (q < lo || q - lo > size || bytes > size - (q - lo))
+    return err.set(STOCHA_EINVAL, "%s holds %zu bytes from the pointer on, the result needs %zu",
+                   name, q < lo || q - lo > size ? (size_t)0 : size - (size_t)(q - lo), bytes);
+  return STOCHA_OK;
+}
+
+// An output of n x m values of elem bytes placed by `where`: STOCHA_HOST or
+// STOCHA_DEVICE only, a byte count that fits, and for STOCHA_DEVICE a device
+// buffer of the current device that holds it.  *bytes = the byte count.
+inline int check_output(const void* out, int where, uint64_t n, uint64_t m, uint64_t elem,
+                        const char* name, size_t* bytes, const Err& err) {
+  if (where != STOCHA_HOST && where != STOCHA_DEVICE)
+    return err.set(STOCHA_EINVAL, "where must be STOCHA_HOST (0) or STOCHA_DEVICE (1), got %d",
+                   where);
+  if (!byte_count(n, m, elem, bytes))
+    return err.set(STOCHA_EINVAL, "%s of %llu x %llu values does not fit in memory", name,
+                   (unsigned long long)n, (unsigned long long)m);
+  if (where == STOCHA_DEVICE && *bytes) {
+    if (int rc = check_device(err)) return rc;
+    return check_device_span(out, *bytes, name, err);
+  }
+  return STOCHA_OK;
+}
+
 struct Workspace {
   void* dev = nullptr;
   size_t dev_bytes = 0;

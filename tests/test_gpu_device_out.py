@@ -121,3 +121,61 @@ def test_rng_device_output_and_state(gpu, n):
         host, dev = _host_and_device(f, n)
         _same_bits(host, dev)
         assert states[(tag, L.STOCHA_HOST)] == states[(tag, L.STOCHA_DEVICE)]
+
+
+@pytest.mark.parametrize("name", ["gbm", "heston_euler", "multi_gbm", "terminal_prices_gbm",
+                                  "gaussian_copula"])
+def test_device_output_must_be_device_memory(gpu, name):
+    # a host array, or pinned host memory, given as a STOCHA_DEVICE output is
+    # refused before any kernel runs (a kernel writing it would fault)
+    import torch
+    call, n = _cases()[name]
+    eb = L.ErrBuf()
+    host = np.full(n, 7.0)
+    rc = call(host.ctypes.data, L.STOCHA_DEVICE, eb.buf, eb.n)
+    assert rc == L.STOCHA_EINVAL and "not device memory" in eb.msg, eb.msg
+    assert (host == 7.0).all()
+    pinned = torch.full((n,), 7.0, dtype=torch.float64).pin_memory()
+    rc = call(pinned.data_ptr(), L.STOCHA_DEVICE, eb.buf, eb.n)
+    assert rc == L.STOCHA_EINVAL and "not device memory" in eb.msg, eb.msg
+    assert bool((pinned == 7.0).all())
+    # the refusal leaves no error behind: the next call runs normally
+    h2, d2 = _host_and_device(call, n)
+    _same_bits(h2, d2)
+
+
+def _segment_of(ptr):
+    """(address, size) of the device allocation torch's caching allocator
+    carved `ptr` from (requests between 1 and 10 MiB share 20 MiB
+    segments, so a tensor's own size says nothing about it)."""
+    import torch
+    for seg in torch.cuda.memory_snapshot():
+        if seg["address"] <= ptr < seg["address"] + seg["total_size"]:
+            return seg["address"], seg["total_size"]
+    return None
+
+
+def test_device_output_too_small_is_refused(gpu):
+    import torch
+    call, n = _cases()["gbm"]
+    buf = torch.full((16 << 17,), 7.0, dtype=torch.float64, device="cuda")  # 16 MiB
+    torch.cuda.synchronize()
+    seg = _segment_of(buf.data_ptr())
+    if seg is None:
+        pytest.skip("the caching allocator's segment of the buffer is not listed")
+    addr, size = seg
+    end = addr + size
+    assert buf.data_ptr() + buf.numel() * 8 <= end
+    eb = L.ErrBuf()
+    # 80 bytes before the end of the allocation: far less than n doubles
+    rc = call(end - 80, L.STOCHA_DEVICE, eb.buf, eb.n)
+    assert rc == L.STOCHA_EINVAL and "the result needs" in eb.msg, eb.msg
+    torch.cuda.synchronize()
+    assert bool((buf == 7.0).all())
+    # at the start of the buffer it fits and matches the host result
+    assert 8 * n <= buf.numel() * 8
+    eb2 = L.ErrBuf()
+    L.check(call(buf.data_ptr(), L.STOCHA_DEVICE, eb2.buf, eb2.n), eb2)
+    host = np.empty(n)
+    L.check(call(host.ctypes.data, L.STOCHA_HOST, eb2.buf, eb2.n), eb2)
+    _same_bits(host, buf[:n].cpu().numpy())
