@@ -222,6 +222,25 @@ int stocha_european_fused(int model, const double *params, uint64_t n_params, do
                           double *price, double *std_err, double *sums, char *err,
                           size_t errlen);
 
+/* stocha_european_fused with nothing allocated.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or pinned host
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Arguments and results are those of
+ * stocha_european_fused, except that price, std_err (n_strikes doubles each)
+ * and sums (2*n_strikes, may be NULL) are DEVICE pointers of the current
+ * device, checked before any launch.  The block partials, totals and the
+ * last-block counter live in a static device array of 524,544 bytes per
+ * device (room for 4096 blocks x 16 sums: every n_paths and n_strikes the
+ * validation admits fits; anything larger is refused with STOCHA_EINVAL).
+ * All calls on one device share that array, so calls that overlap on one
+ * device race.  Returns after the results are written (stream synchronized).
+ */
+int stocha_european_fused_noalloc(int model, const double *params, uint64_t n_params, double t,
+                                  uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
+                                  int schedule, const double *strikes, uint64_t n_strikes,
+                                  double r, double *price, double *std_err, double *sums,
+                                  char *err, size_t errlen);
+
 /* LSMC with a choice of path precision (FP32 paths: compensated FP64
  * moment sums, FP64 solve).  Same validation as stocha_lsmc_american_option. */
 int stocha_lsmc_ex(double s0, double k, double r, double sigma, double t, uint64_t steps,

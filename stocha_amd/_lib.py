@@ -69,6 +69,8 @@ _SIGS = {
     "stocha_student_t_copula": ([_pd, _u, _u, _d, _u, _u, _vp, _i, _ps, _sz], C.c_int),
     "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _pd, _pd, _pd,
                                _ps, _sz], C.c_int),
+    "stocha_european_fused_noalloc": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _vp,
+                                       _vp, _vp, _ps, _sz], C.c_int),
     "stocha_lsmc_ex": ([_d] * 5 + [_u, _u, _i, _u, _u, _i, _pd, _pd, _ps, _sz], C.c_int),
     "stocha_basket": ([_pd, _pd, _pd, _pd, _pd, _u, _d, _d, _d, _u, _u, _u, _i, _i, _pd, _pd,
                        _pd, _pd, _pd, _ps, _sz], C.c_int),

@@ -710,12 +710,22 @@ int stocha_terminal_prices(int model, const double* params, uint64_t n_params, c
   return STOCHA_OK;
 }
 
-int stocha_european_fused(int model, const double* params, uint64_t n_params, double t,
-                          uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
-                          int schedule, const double* strikes, uint64_t n_strikes, double r,
-                          double* price, double* std_err, double* sums, char* errbuf,
-                          size_t errlen) {
-  Err err{errbuf, errlen};
+}  // extern "C"
+
+namespace {
+
+// Validation and per-launch constants shared by the two fused entry points.
+struct FusedSetup {
+  GbmConst gc;
+  HestonConst hc;
+  MertonConst mc;
+  const void* cptr = nullptr;
+  StreamDesc sd;
+};
+
+int fused_validate(int model, const double* params, uint64_t n_params, double t,
+                   uint64_t steps, uint64_t n_paths, int precision, int schedule,
+                   uint64_t n_strikes, const Err& err) {
   static const uint64_t np[3] = {3, 7, 6};
   if (model < 0 || model > 2) return err.set(STOCHA_EINVAL, "unknown model id %d", model);
   if (n_params != np[model])
@@ -731,26 +741,43 @@ int stocha_european_fused(int model, const double* params, uint64_t n_params, do
   if (t <= 0.0) return err.set(STOCHA_EINVAL, "t must be positive");
   if (steps == 0 || n_paths == 0)
     return err.set(STOCHA_EINVAL, "steps and n_paths must be positive");
+  return STOCHA_OK;
+}
+
+void fused_setup(int model, const double* params, double t, uint64_t steps, uint64_t seed,
+                 FusedSetup* f) {
+  if (model == kGbm) {
+    f->gc = gbm_const(params[0], params[1], params[2], t, steps, 0.0);
+    f->cptr = &f->gc;
+  } else if (model == kHeston) {
+    f->hc = heston_const(params, t, steps);
+    f->cptr = &f->hc;
+  } else {
+    f->mc = merton_const(params, t, steps);
+    f->cptr = &f->mc;
+  }
+  make_stream_desc(seed, block_size(model, steps), &f->sd);
+}
+
+}  // namespace
+
+extern "C" {
+
+int stocha_european_fused(int model, const double* params, uint64_t n_params, double t,
+                          uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
+                          int schedule, const double* strikes, uint64_t n_strikes, double r,
+                          double* price, double* std_err, double* sums, char* errbuf,
+                          size_t errlen) {
+  Err err{errbuf, errlen};
+  if (int rc = fused_validate(model, params, n_params, t, steps, n_paths, precision, schedule,
+                              n_strikes, err))
+    return rc;
   std::lock_guard<std::mutex> lk(globals().mu);
   if (int rc = check_device(err)) return rc;
-  GbmConst gc;
-  HestonConst hc;
-  MertonConst mc;
-  const void* cptr;
-  if (model == kGbm) {
-    gc = gbm_const(params[0], params[1], params[2], t, steps, 0.0);
-    cptr = &gc;
-  } else if (model == kHeston) {
-    hc = heston_const(params, t, steps);
-    cptr = &hc;
-  } else {
-    mc = merton_const(params, t, steps);
-    cptr = &mc;
-  }
-  StreamDesc sd;
-  make_stream_desc(seed, block_size(model, steps), &sd);
+  FusedSetup f;
+  fused_setup(model, params, t, steps, seed, &f);
   double h[16];
-  if (int rc = launch_fused(model, precision, schedule, cptr, sd, n_paths, steps, strikes,
+  if (int rc = launch_fused(model, precision, schedule, f.cptr, f.sd, n_paths, steps, strikes,
                             (int)n_strikes, h, globals().stream, err))
     return rc;
   const double disc = exp(-r * t);
@@ -767,6 +794,31 @@ int stocha_european_fused(int model, const double* params, uint64_t n_params, do
     }
   }
   return STOCHA_OK;
+}
+
+// stocha_european_fused with nothing allocated: device outputs, partials in
+// a static device array of fused.hip.  No reference counterpart.
+int stocha_european_fused_noalloc(int model, const double* params, uint64_t n_params, double t,
+                                  uint64_t steps, uint64_t n_paths, uint64_t seed, int precision,
+                                  int schedule, const double* strikes, uint64_t n_strikes,
+                                  double r, double* d_price, double* d_std_err, double* d_sums,
+                                  char* errbuf, size_t errlen) {
+  Err err{errbuf, errlen};
+  if (int rc = fused_validate(model, params, n_params, t, steps, n_paths, precision, schedule,
+                              n_strikes, err))
+    return rc;
+  std::lock_guard<std::mutex> lk(globals().mu);
+  if (int rc = check_device(err)) return rc;
+  const size_t nb = sizeof(double) * n_strikes;
+  if (int rc = check_device_span(d_price, nb, "price", err)) return rc;
+  if (int rc = check_device_span(d_std_err, nb, "std_err", err)) return rc;
+  if (d_sums)
+    if (int rc = check_device_span(d_sums, 2 * nb, "sums", err)) return rc;
+  FusedSetup f;
+  fused_setup(model, params, t, steps, seed, &f);
+  return launch_fused_noalloc(model, precision, schedule, f.cptr, f.sd, n_paths, steps, strikes,
+                              (int)n_strikes, exp(-r * t), d_price, d_std_err, d_sums,
+                              globals().stream, err);
 }
 
 // Basket call on correlated GBM (multi_gbm.rs streams) with pathwise adjoint

@@ -171,26 +171,40 @@ int launch_reduce_partials(const double* part, int n_blocks, int na, double* out
 template <class Model, class Src, int kQueue, class Const, bool kFp32, class Payoff>
 static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
                         const Payoff& pay, int na, double* h_out, int schedule, hipStream_t st,
-                        const Err& err) {
+                        const Err& err, const ScratchTarget* into = nullptr) {
   if (steps > 0xffffffffull) return err.set(STOCHA_EINVAL, "steps must be below 2^32");
   const unsigned grid = fused_grid(n_paths);
+  double* part;
+  unsigned* counter;
+  double* out;
   Workspace* ws = nullptr;
-  const size_t part_bytes = sizeof(double) * grid * na;
-  if (int rc = workspace(part_bytes, sizeof(double) * kMaxAcc, &ws, err)) return rc;
-  double* part = static_cast<double*>(ws->dev);
-  // the last block writes the totals straight into the pinned host buffer
-  // (device-visible through unified addressing; read after the stream sync)
-  double* out = ws->host;
+  if (into) {  // caller-placed partials, counter and totals (nothing allocated)
+    if ((size_t)grid * na > into->max_partials)
+      return err.set(STOCHA_EINVAL, "%u blocks x %d sums exceed the %zu partials of the scratch",
+                     grid, na, into->max_partials);
+    part = into->partials;
+    counter = into->counter;
+    out = into->totals;
+  } else {
+    const size_t part_bytes = sizeof(double) * grid * na;
+    if (int rc = workspace(part_bytes, sizeof(double) * kMaxAcc, &ws, err)) return rc;
+    part = static_cast<double*>(ws->dev);
+    counter = ws->counter;
+    // the last block writes the totals straight into the pinned host buffer
+    // (device-visible through unified addressing; read after the stream sync)
+    out = ws->host;
+  }
   KernelTimer timer(st);
   timer.start();
   if (schedule == kScheduleLoop)
     terminal_payoff_kernel<Model, Const, kFp32, Payoff><<<grid, kFusedBlock, 0, st>>>(
-        mc, sd, n_paths, steps, pay, part, ws->counter, out);
+        mc, sd, n_paths, steps, pay, part, counter, out);
   else
     terminal_feed_kernel<Src, Model, Const, Payoff, kQueue><<<grid, kFusedBlock, 0, st>>>(
-        mc, sd, n_paths, (uint32_t)steps, pay, part, ws->counter, out);
+        mc, sd, n_paths, (uint32_t)steps, pay, part, counter, out);
   STOCHA_HIP(hipGetLastError());
   timer.stop(1);
+  if (into) return STOCHA_OK;
   STOCHA_HIP(hipStreamSynchronize(st));
   for (int i = 0; i < na; ++i) h_out[i] = ws->host[i];
   return STOCHA_OK;
@@ -198,7 +212,8 @@ static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths,
 
 int launch_fused(int model, int precision, int schedule, const void* mconst,
                  const StreamDesc& sd, uint64_t n_paths, uint64_t steps, const double* strikes,
-                 int n_strikes, double* h_sums, hipStream_t st, const Err& err) {
+                 int n_strikes, double* h_sums, hipStream_t st, const Err& err,
+                 const ScratchTarget* into) {
   CallPayoff pay{};
   pay.n = n_strikes;
   for (int i = 0; i < n_strikes; ++i) pay.k[i] = strikes[i];
@@ -207,22 +222,108 @@ int launch_fused(int model, int precision, int schedule, const void* mconst,
   switch (model) {
     case kGbm: {
       const GbmConst& c = *(const GbmConst*)mconst;
-      return f32 ? run_terminal<Gbm32, GbmSrc<true>, 16, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
-                 : run_terminal<Gbm64, GbmSrc<false>, 8, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
+      return f32 ? run_terminal<Gbm32, GbmSrc<true>, 16, GbmConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err, into)
+                 : run_terminal<Gbm64, GbmSrc<false>, 8, GbmConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err, into);
     }
     case kHeston: {
       const HestonConst& c = *(const HestonConst*)mconst;
-      return f32 ? run_terminal<Heston32, HestonSrc<true>, 16, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
-                 : run_terminal<Heston64, HestonSrc<false>, 8, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
+      return f32 ? run_terminal<Heston32, HestonSrc<true>, 16, HestonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err, into)
+                 : run_terminal<Heston64, HestonSrc<false>, 8, HestonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err, into);
     }
     case kMerton: {
       const MertonConst& c = *(const MertonConst*)mconst;
-      return f32 ? run_terminal<Merton32, MertonSrc<true, Merton32>, 8, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err)
-                 : run_terminal<Merton64, MertonSrc<false, Merton64>, 4, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err);
+      return f32 ? run_terminal<Merton32, MertonSrc<true, Merton32>, 8, MertonConst, true>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err, into)
+                 : run_terminal<Merton64, MertonSrc<false, Merton64>, 4, MertonConst, false>(c, sd, n_paths, steps, pay, na, h_sums, schedule, st, err, into);
     }
     default:
       return err.set(STOCHA_EINVAL, "unknown model id %d", model);
   }
+}
+
+// ---------------------------------------------------------------------------
+// The no-allocation form of the fused European call
+// (stocha_european_fused_noalloc).  Its only device memory beyond the
+// caller's inputs and outputs is `scratch`, a file-scope device array holding
+// the block partials (fused_grid's at most 4096 blocks x kMaxAcc sums), the
+// totals and the last-block counter: 512.25 KiB on each device the library
+// runs on, held until the process ends.  Host code reaches it only through
+// the address hipGetSymbolAddress returns for the current device (kept per
+// device); the array's name is a host-side stand-in, not a device address.
+// Every call on a device shares that device's array, so calls that overlap on
+// one device race.
+constexpr size_t kScratchPartials = (size_t)4096 * kMaxAcc;
+constexpr size_t kScratchTotals = kScratchPartials;           // kMaxAcc doubles
+constexpr size_t kScratchCounter = kScratchTotals + kMaxAcc;  // one uint32 (a double slot)
+constexpr size_t kScratchDoubles = kScratchCounter + 16;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static_assert(kScratchDoubles * sizeof(double) == 524544, "stated scratch size");
+
+constexpr int kScratchDevices = 64;
+
+// Device address of `scratch` on the current device, looked up once per
+// device (callers hold the library mutex).
+static int scratch_address(double** out, const Err& err) {
+  static double* addr[kScratchDevices] = {};
+  int dev = 0;
+  STOCHA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return err.set(STOCHA_EDEVICE, "device %d is beyond the %d devices of the scratch table", dev,
+                   kScratchDevices);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    STOCHA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return err.set(STOCHA_EDEVICE, "no device address for the scratch array");
+    addr[dev] = static_cast<double*>(p);
+  }
+  *out = addr[dev];
+  return STOCHA_OK;
+}
+
+__global__ void clear_counter_kernel(unsigned* __restrict__ counter) {
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+// Prices, standard errors and sums from the totals, with the host's
+// expressions of stocha_european_fused (same operations, same order).
+__global__ void fused_epilogue_kernel(const double* __restrict__ tot, const int n_strikes,
+                                      const double disc, const double n,
+                                      double* __restrict__ price, double* __restrict__ std_err,
+                                      double* __restrict__ sums) {
+  const int k = threadIdx.x;
+  if (k >= n_strikes) return;
+  const double s1 = tot[2 * k], s2 = tot[2 * k + 1];
+  price[k] = disc * s1 / n;
+  double var = n > 1 ? (s2 - s1 * s1 / n) / (n - 1.0) : 0.0;
+  var = var > 0.0 ? var : 0.0;
+  std_err[k] = disc * sqrt(var / n);
+  if (sums) {
+    sums[2 * k] = s1;
+    sums[2 * k + 1] = s2;
+  }
+}
+
+int launch_fused_noalloc(int model, int precision, int schedule, const void* mconst,
+                         const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
+                         const double* strikes, int n_strikes, double disc, double* d_price,
+                         double* d_std_err, double* d_sums, hipStream_t st, const Err& err) {
+  double* base = nullptr;
+  if (int rc = scratch_address(&base, err)) return rc;
+  ScratchTarget into;
+  into.partials = base;
+  into.max_partials = kScratchPartials;
+  into.totals = base + kScratchTotals;
+  into.counter = reinterpret_cast<unsigned*>(base + kScratchCounter);
+  // the counter must start at zero (a fill kernel rather than a memset node)
+  clear_counter_kernel<<<1, 64, 0, st>>>(into.counter);
+  STOCHA_HIP(hipGetLastError());
+  if (int rc = launch_fused(model, precision, schedule, mconst, sd, n_paths, steps, strikes,
+                            n_strikes, nullptr, st, err, &into))
+    return rc;
+  fused_epilogue_kernel<<<1, 64, 0, st>>>(into.totals, n_strikes, disc, (double)n_paths, d_price,
+                                          d_std_err, d_sums);
+  STOCHA_HIP(hipGetLastError());
+  STOCHA_HIP(hipStreamSynchronize(st));
+  return STOCHA_OK;
 }
 
 int run_greeks_pathwise(const GbmConst& gc, const StreamDesc& sd, uint64_t n_paths,

@@ -240,9 +240,27 @@ int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n
                  const Err& err);
 // fused.hip
 enum Schedule : int { kScheduleFeed = 0, kScheduleLoop = 1 };
+// Device memory a fused launch reduces into instead of the workspace: block
+// partials (room for max_partials doubles), a zeroed counter and the totals.
+struct ScratchTarget {
+  double* partials;
+  size_t max_partials;
+  unsigned* counter;
+  double* totals;
+};
+// With `into`, the launch only enqueues on st and h_sums is unused; without,
+// it reduces through the workspace and returns the sums in h_sums.
 int launch_fused(int model, int precision, int schedule, const void* mconst,
                  const StreamDesc& sd, uint64_t n_paths, uint64_t steps, const double* strikes,
-                 int n_strikes, double* h_sums, hipStream_t st, const Err& err);
+                 int n_strikes, double* h_sums, hipStream_t st, const Err& err,
+                 const ScratchTarget* into = nullptr);
+// The fused calls with nothing allocated: partials in the file's static
+// device scratch, prices / standard errors / sums (d_sums may be null)
+// written to the caller's device arrays; returns after the stream sync.
+int launch_fused_noalloc(int model, int precision, int schedule, const void* mconst,
+                         const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
+                         const double* strikes, int n_strikes, double disc, double* d_price,
+                         double* d_std_err, double* d_sums, hipStream_t st, const Err& err);
 // greeks_fd scenarios on shared draws: consts is an array of n (<= 8)
 // GbmConst / HestonConst / MertonConst whose draw grammar is the same (same
 // steps; for Merton the same lambda*dt); payoff max(sign*(S_T - strike), 0);
