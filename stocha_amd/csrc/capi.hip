@@ -15,6 +15,7 @@
 
 #include "host.hpp"
 #include "multi.hpp"
+#include "sums.hpp"
 
 namespace stocha {
 
@@ -32,6 +33,10 @@ int workspace(size_t dev_bytes, size_t host_bytes, Workspace** out, const Err& e
   if (!w.counter) {
     STOCHA_HIP(hipMalloc((void**)&w.counter, sizeof(unsigned) * kCounters));
     STOCHA_HIP(hipMemset(w.counter, 0, sizeof(unsigned) * kCounters));
+  }
+  if (!w.sums_part) {
+    STOCHA_HIP(hipMalloc((void**)&w.sums_part, sizeof(double) * kSumsPartials));
+    if (int rc = fill_empty_slots(w.sums_part, kSumsPartials, globals().stream, err)) return rc;
   }
   if (dev_bytes > w.dev_bytes) {
     if (w.dev) STOCHA_HIP(hipFree(w.dev));

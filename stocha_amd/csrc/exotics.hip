@@ -218,12 +218,11 @@ int run_sum(const GbmConst& c, const F& f, uint64_t seed, uint64_t n_paths, uint
   make_stream_desc(seed, block_size(kGbm, steps), &sd);
   const unsigned grid = fused_grid(n_paths);
   Workspace* ws = nullptr;
-  if (int rc = workspace(sizeof(double) * grid, sizeof(double) * kMaxAcc, &ws, err)) return rc;
+  if (int rc = workspace(0, sizeof(double) * kMaxAcc, &ws, err)) return rc;
   KernelTimer timer(st);
   timer.start();
   exotic_sum_kernel<F><<<grid, kFusedBlock, 0, st>>>(c, f, sd, n_paths, (uint32_t)steps,
-                                                     static_cast<double*>(ws->dev), ws->counter,
-                                                     ws->host);
+                                                     ws->sums_part, ws->counter, ws->host);
   STOCHA_HIP(hipGetLastError());
   timer.stop(1);
   STOCHA_HIP(hipStreamSynchronize(st));

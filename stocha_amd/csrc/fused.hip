@@ -167,6 +167,20 @@ int launch_reduce_partials(const double* part, int n_blocks, int na, double* out
   return STOCHA_OK;
 }
 
+__global__ __launch_bounds__(256) void fill_empty_kernel(uint64_t* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    p[i] = kEmptySlot;
+}
+
+int fill_empty_slots(double* p, size_t n, hipStream_t st, const Err& err) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks == 0) return STOCHA_OK;
+  fill_empty_kernel<<<(unsigned)blocks, 256, 0, st>>>(reinterpret_cast<uint64_t*>(p), n);
+  STOCHA_HIP(hipGetLastError());
+  return STOCHA_OK;
+}
+
 
 template <class Model, class Src, int kQueue, class Const, bool kFp32, class Payoff>
 static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths, uint64_t steps,
@@ -186,9 +200,8 @@ static int run_terminal(const Const& mc, const StreamDesc& sd, uint64_t n_paths,
     counter = into->counter;
     out = into->totals;
   } else {
-    const size_t part_bytes = sizeof(double) * grid * na;
-    if (int rc = workspace(part_bytes, sizeof(double) * kMaxAcc, &ws, err)) return rc;
-    part = static_cast<double*>(ws->dev);
+    if (int rc = workspace(0, sizeof(double) * kMaxAcc, &ws, err)) return rc;
+    part = ws->sums_part;
     counter = ws->counter;
     // the last block writes the totals straight into the pinned host buffer
     // (device-visible through unified addressing; read after the stream sync)
@@ -244,7 +257,8 @@ int launch_fused(int model, int precision, int schedule, const void* mconst,
 // The no-allocation form of the fused European call
 // (stocha_european_fused_noalloc).  Its only device memory beyond the
 // caller's inputs and outputs is `scratch`, a file-scope device array holding
-// the block partials (fused_grid's at most 4096 blocks x kMaxAcc sums), the
+// the block-partial slots (fused_grid's at most 4096 blocks x kMaxAcc sums,
+// set empty once per device at set-up and left empty by every launch), the
 // totals and the last-block counter: 512.25 KiB on each device the library
 // runs on, held until the process ends.  Host code reaches it only through
 // the address hipGetSymbolAddress returns for the current device (kept per
@@ -273,6 +287,10 @@ static int scratch_address(double** out, const Err& err) {
     void* p = nullptr;
     STOCHA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
     if (!p) return err.set(STOCHA_EDEVICE, "no device address for the scratch array");
+    // the partial slots start empty; every reduction leaves them so
+    if (int rc = fill_empty_slots(static_cast<double*>(p), kScratchPartials, nullptr, err))
+      return rc;
+    STOCHA_HIP(hipStreamSynchronize(nullptr));
     addr[dev] = static_cast<double*>(p);
   }
   *out = addr[dev];
@@ -449,14 +467,13 @@ static int run_fd(const Const* c, int n, double strike, double sign, const Strea
   sc.sign = sign;
   const unsigned grid = fused_grid(n_paths);
   Workspace* ws = nullptr;
-  if (int rc = workspace(sizeof(double) * grid * n, sizeof(double) * kMaxAcc, &ws, err))
-    return rc;
+  if (int rc = workspace(0, sizeof(double) * kMaxAcc, &ws, err)) return rc;
   KernelTimer timer(st);
   timer.start();
   constexpr int kWaves =
       std::is_same<Model, Heston64>::value || std::is_same<Model, HestonQe64>::value ? 3 : 4;
   fd_feed_kernel<Src, Model, Const, kRing64<Src>, kWaves><<<grid, kFusedBlock, 0, st>>>(
-      sc, sd, n_paths, (uint32_t)steps, static_cast<double*>(ws->dev), ws->counter, ws->host);
+      sc, sd, n_paths, (uint32_t)steps, ws->sums_part, ws->counter, ws->host);
   STOCHA_HIP(hipGetLastError());
   timer.stop(1);
   STOCHA_HIP(hipStreamSynchronize(st));

@@ -84,6 +84,8 @@ struct DevBuf {
 //   dev     device scratch (block partials, results, paths)
 //   counter kCounters zeroed uint32s: [0] the last-block reductions of the
 //           fused kernels; every kernel leaves them as it found them
+//   sums_part  the fused reductions' block-partial slots (sums.hpp), empty
+//           between launches
 //   host    pinned host memory for results copied back
 // A loud error when no device is visible (the library has no CPU path).
 inline int check_device(const Err& err) {
@@ -158,9 +160,13 @@ struct Workspace {
   void* dev = nullptr;
   size_t dev_bytes = 0;
   unsigned* counter = nullptr;
+  double* sums_part = nullptr;  // kSumsPartials slots, empty between launches
   double* host = nullptr;
   size_t host_bytes = 0;
 };
+// n slots set to the empty-slot pattern of the fused reductions (a fill
+// kernel on st).
+int fill_empty_slots(double* p, size_t n, hipStream_t st, const Err& err);
 constexpr int kCounters = 1024;
 // Workspace of the current device with at least the given sizes.
 int workspace(size_t dev_bytes, size_t host_bytes, Workspace** out, const Err& err);

@@ -20,6 +20,7 @@
 
 #include "host.hpp"
 #include "drivers.hpp"
+#include "sums.hpp"
 
 namespace stocha {
 
@@ -77,13 +78,7 @@ __global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, c
                                                             const uint64_t n, const uint32_t steps,
                                                             void* __restrict__ paths,
                                                             double* __restrict__ cf, const double k,
-                                                            const int is_put,
-                                                            unsigned* __restrict__ flags,
-                                                            const unsigned n_flags) {
-  // the cooperative induction that follows starts from zero barrier flags
-  if (blockIdx.x == 0)
-    for (unsigned j = threadIdx.x; j < n_flags; j += kLsBlock)
-      __hip_atomic_store(flags + j * 4, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                                            const int is_put) {
   using Src = GbmSrc<kFp32>;
   using Model = typename std::conditional<kFp32, Gbm32, Gbm64>::type;
   constexpr uint32_t kSlots = kFp32 ? 16 : 8;
@@ -422,54 +417,18 @@ constexpr int kCoopBlock = 1024;
 constexpr int kCoopWaves = kCoopBlock / 64;
 constexpr int kCoopMaxPpt = 16;
 
-// Cross-block data of the cooperative kernel (partials, barrier words) is
-// only ever accessed with agent-scope relaxed atomics: on gfx950 these are
-// plain loads/stores with the sc1 bit, coherent across the XCDs' L2s, so no
-// L2 write-back / invalidate (the cost of __threadfence, done per wave) is
-// needed; s_waitcnt orders them.  cooperative_groups' grid.sync() costs
-// ~33 us per barrier at 256 blocks on MI355X, this barrier ~3 us.
-template <class T>
-__device__ __forceinline__ T ld_agent(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Cross-block data of the cooperative kernel: each date's block partials.
+// They go to slots of their own per date (never reused within a launch) that
+// hold kEmptySlot when the kernel starts; a block stores its partial with a
+// relaxed agent-scope store (coherent across the XCDs' L2s) and every block
+// reads each slot with await_slot, which waits until the slot is filled.
+// The slots are the flags: no grid barrier, no ordering between different
+// locations, no L2 write-back or invalidate (cooperative groups' grid.sync()
+// costs ~33 us per barrier at 256 blocks on MI355X).  Every block of the
+// cooperative grid is resident, so every awaited slot gets filled.
 template <class T>
 __device__ __forceinline__ void st_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid barrier for a co-resident grid (cooperative launch).  flags: one
-// 16-byte slot per block, zero when the kernel starts (the path generation
-// kernel before it clears them).  Barrier k (1, 2, ...) of the launch: each
-// block stores k in its slot, once its partial stores have completed, and
-// wave 0 of each block reads every slot until all hold at least k.  Plain
-// stores to separate slots: no atomic queues on one address, and arriving
-// and release are one store and one load apart.
-__device__ __forceinline__ void grid_barrier(unsigned* flags, unsigned& k) {
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's partial stores have completed
-  __syncthreads();
-  k = k + 1;
-  if (threadIdx.x < 64) {
-    const unsigned nb = gridDim.x, lane = threadIdx.x;
-    if (lane == 0) st_agent(flags + blockIdx.x * 4, k);
-    for (;;) {
-      bool wait = false;
-      for (unsigned b0 = 0; b0 < nb; b0 += 256) {  // 4 loads in flight per lane
-        unsigned v[4];
-        bool in[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {  // unpredicated loads (slot 0 past nb)
-          const unsigned b = b0 + 64 * j + lane;
-          in[j] = b < nb;
-          v[j] = ld_agent(flags + (in[j] ? b : 0u) * 4);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) wait = wait | (in[j] & (v[j] < k));
-      }
-      if (__builtin_amdgcn_ballot_w64(wait) == 0) break;
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
-  __syncthreads();
 }
 
 struct LsmcCoopArgs {
@@ -478,9 +437,9 @@ struct LsmcCoopArgs {
   uint32_t steps;
   int is_put, ncols;
   double k, discount;
-  double* part;   // [2][grid][kMaxAccB]
-  double* out;    // [2]: sum cf, sum (cf - mean)^2
-  unsigned* bar;  // grid barrier flags (16 bytes per block), zero at kernel start
+  double* part;  // [steps + 1][kMaxAccB][grid] partial slots, empty at kernel start:
+                 // slot `date` for dates 1..steps-1, 0 and `steps` for the two final sums
+  double* out;   // [2]: sum cf, sum (cf - mean)^2
 };
 
 // Moments of x = (S - mean) / sdv = al u + be from the raw moments of
@@ -666,10 +625,16 @@ __device__ __forceinline__ void coop_total(const double* part, int na, double* t
     double s = 0.0;
     for (unsigned b0 = 0; b0 < nb; b0 += 256) {  // 4 loads in flight per lane
       double v[4];
+      uint64_t bits[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {  // unpredicated loads (block 0 past nb)
+        const unsigned b = b0 + 64 * k + lane;
+        bits[k] = load_slot(&col[b < nb ? b : 0u]);
+      }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const unsigned b = b0 + 64 * k + lane;
-        v[k] = b < nb ? ld_agent(&col[b]) : 0.0;
+        v[k] = b < nb ? slot_value(const_cast<double*>(&col[b]), bits[k]) : 0.0;
       }
       s += (v[0] + v[1]) + (v[2] + v[3]);
     }
@@ -708,7 +673,6 @@ __device__ __forceinline__ double exercise(double cf, double s, double sgn, doub
 
 template <bool kFp32, int kPpt>
 __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArgs a) {
-  unsigned gen = 0;  // grid barriers passed
   __shared__ double red[kCoopWaves][kMaxAccB];
   __shared__ double tot[kMaxAccB];
   __shared__ double coef_s[kMaxCols];
@@ -826,10 +790,10 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
         }
       }
     }
-    double* pp = a.part + (date & 1u) * pstride;
+    double* pp = a.part + (uint64_t)date * pstride;
     const double bs = coop_block_sum<kCoopWaves>(acc, na, red);
     if (threadIdx.x < na) st_agent(&pp[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x], bs);
-    grid_barrier(a.bar, gen);
+    __syncthreads();
     coop_total(pp, na, tot);
     // The spots of date-1 are loaded here: after the totals' loads (vector
     // memory returns in order), and ahead of the solve and the next exercise
@@ -899,10 +863,9 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
       acc1[0] += cf[q][tx];
     }
   }
-  // parity-0 buffer: its last reads (date 2) finished before the barrier of date 1
   double s1 = coop_block_sum<kCoopWaves>(acc1, 1, red);
-  if (threadIdx.x == 0) st_agent(&a.part[blockIdx.x], s1);
-  grid_barrier(a.bar, gen);
+  if (threadIdx.x == 0) st_agent(&a.part[blockIdx.x], s1);  // slot 0
+  __syncthreads();
   coop_total(a.part, 1, tot);
   const double mean = tot[0] / (double)a.n;
   double acc2[kMaxAccB] = {};
@@ -914,13 +877,13 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
       acc2[0] += dl * dl;
     }
   }
-  // parity-1 buffer: its last reads (date 1) finished before the barrier above
   double s2 = coop_block_sum<kCoopWaves>(acc2, 1, red);
-  if (threadIdx.x == 0) st_agent(&a.part[pstride + blockIdx.x], s2);
+  double* fin = a.part + (uint64_t)a.steps * pstride;  // slot `steps`
+  if (threadIdx.x == 0) st_agent(&fin[blockIdx.x], s2);
   const double sum_cf = tot[0];
-  grid_barrier(a.bar, gen);
+  __syncthreads();
   if (blockIdx.x == 0) {
-    coop_total(a.part + pstride, 1, tot);
+    coop_total(fin, 1, tot);
     if (threadIdx.x == 0) {
       a.out[0] = sum_cf;
       a.out[1] = tot[0];
@@ -969,18 +932,17 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
   STOCHA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const bool coop = steps >= 2 && n <= (uint64_t)cus * kCoopBlock * kCoopMaxPpt &&
                     !getenv("STOCHA_LSMC_MULTIKERNEL");
-  // workspace: paths | cf | partials | per-date state | 2 results (256-byte aligned)
+  // workspace: paths | cf | partials | per-date state | 2 results (256-byte aligned);
+  // the cooperative induction's partials are one slot set per date
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t part_n = coop ? 2 * (size_t)cus * kMaxAccB : (size_t)nb * kMaxAccB;
+  const size_t part_n =
+      coop ? (size_t)(steps + 1) * (size_t)cus * kMaxAccB : (size_t)nb * kMaxAccB;
   const size_t o_cf = up(esz * n * steps);
   const size_t o_part = o_cf + up(sizeof(double) * n);
   const size_t o_ds = o_part + up(sizeof(double) * part_n);
   const size_t o_scal = o_ds + up(sizeof(DateState) * (steps + 1));
-  const size_t o_flag = o_scal + 256;
-  const unsigned n_flags = coop ? (unsigned)cus : 0u;
   Workspace* ws = nullptr;
-  if (int rc = workspace(o_flag + up(16 * (size_t)n_flags), 2 * sizeof(double), &ws, err))
-    return rc;
+  if (int rc = workspace(o_scal + 256, 2 * sizeof(double), &ws, err)) return rc;
   char* base = static_cast<char*>(ws->dev);
   void* paths = base;
   double* cf = reinterpret_cast<double*>(base + o_cf);
@@ -990,11 +952,12 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
 
   KernelTimer timer(st);
   timer.start();
-  unsigned* flags = reinterpret_cast<unsigned*>(base + o_flag);
   // the cooperative induction takes its cash flows from the last column
-  // itself (no cf array)
+  // itself (no cf array) and starts from empty partial slots
+  if (coop)
+    if (int rc = fill_empty_slots(part, part_n, st, err)) return rc;
   lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, (uint32_t)steps, paths,
-                                                  coop ? nullptr : cf, k, is_put, flags, n_flags);
+                                                  coop ? nullptr : cf, k, is_put);
   STOCHA_HIP(hipGetLastError());
   if (coop) {
     LsmcCoopArgs ca;
@@ -1007,7 +970,6 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
     ca.discount = discount;
     ca.part = part;
     ca.out = scal;
-    ca.bar = flags;
     if (int rc = lsmc_coop<kFp32>(ca, (unsigned)cus, st, err)) return rc;
     STOCHA_HIP(hipGetLastError());
   } else {

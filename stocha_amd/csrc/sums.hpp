@@ -35,35 +35,90 @@ __device__ __forceinline__ double block_total(const double* acc, int na,
   return s;
 }
 
+// Bit pattern of an empty partial slot: a signalling NaN, which no
+// arithmetic produces (NaN results are quiet), so no block sum can equal it.
+constexpr uint64_t kEmptySlot = 0x7ff4deadbeef0001ull;
+// Partial slots a fused reduction may use: fused_grid's at most 4096 blocks x
+// kMaxAcc sums.
+constexpr size_t kSumsPartials = (size_t)4096 * kMaxAcc;
+
+// A slot's value once its writer's store is visible: relaxed agent-scope
+// loads of the slot itself (coherent across the XCDs' L2s) until it no
+// longer holds kEmptySlot.  The slot is the flag, so no ordering between
+// different locations is needed and no cache write-back or invalidate.
+// Every awaited slot has a writer that has already run or is resident, so
+// the wait is microseconds; the bound (2^24 polls, ~0.5 s of sleeps) only
+// guarantees that the kernel ends even if that ever failed -- the result is
+// then a NaN, never a hang.
+__device__ __forceinline__ double await_slot(double* p) {
+  uint64_t* q = reinterpret_cast<uint64_t*>(p);
+  for (uint32_t spins = 0; spins < (1u << 24); ++spins) {
+    const uint64_t v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v != kEmptySlot) return __longlong_as_double((long long)v);
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return __builtin_nan("");
+}
+
+// One relaxed agent-scope load of a slot (its raw bits): issue a row of
+// these together, then settle each with slot_value, so the loads overlap and
+// only a slot still empty costs a wait.
+__device__ __forceinline__ uint64_t load_slot(const double* p) {
+  return __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double slot_value(double* p, uint64_t bits) {
+  return bits != kEmptySlot ? __longlong_as_double((long long)bits) : await_slot(p);
+}
+
+__device__ __forceinline__ void put_slot(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void empty_slot(double* p) {
+  __hip_atomic_store(reinterpret_cast<uint64_t*>(p), kEmptySlot, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // End of a fused kernel: store the block partial; the last block to arrive
 // reduces all partials in a fixed order into out[0..na) and re-zeroes the
-// counter (every block has incremented it by then).  The partials are
-// written and read with agent-scope relaxed atomics (sc1 loads/stores,
-// coherent across the XCDs' L2s) and ordered by s_waitcnt, so no block pays
-// for an L2 write-back / invalidate (__threadfence).
+// counter (every block has incremented it by then).  partials: slots that
+// hold kEmptySlot when the kernel starts (at least gridDim.x * na of them);
+// each block fills its own, the last block (the counter tells which) waits
+// for each slot to be filled (await_slot), adds it and empties it again for
+// the next launch.  The counter only elects the last block: it orders
+// nothing, so a relaxed increment suffices.
 __device__ __forceinline__ void finish_sums(const double* acc, int na, double (*red)[kMaxAcc],
                                             double* partials, unsigned* __restrict__ counter,
                                             double* __restrict__ out) {
   __shared__ bool last;
   const double s = block_total(acc, na, red);
-  if (threadIdx.x < na)
-    __hip_atomic_store(&partials[(uint64_t)blockIdx.x * na + threadIdx.x], s, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  __builtin_amdgcn_s_waitcnt(0);  // the partial store has completed
+  if (threadIdx.x < na) put_slot(&partials[(uint64_t)blockIdx.x * na + threadIdx.x], s);
   __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  if (threadIdx.x == 0)
+    last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           gridDim.x - 1;
   __syncthreads();
   if (!last) return;
   double tot[kMaxAcc];
 #pragma unroll
   for (int i = 0; i < kMaxAcc; ++i) tot[i] = 0.0;
   for (unsigned b = threadIdx.x; b < gridDim.x; b += kFusedBlock) {
+    double* row = &partials[(uint64_t)b * na];
+    uint64_t bits[kMaxAcc];
+#pragma unroll
+    for (int i = 0; i < kMaxAcc; ++i) bits[i] = i < na ? load_slot(row + i) : 0;
 #pragma unroll
     for (int i = 0; i < kMaxAcc; ++i)
-      if (i < na)
-        tot[i] += __hip_atomic_load(&partials[(uint64_t)b * na + i], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+      if (i < na) tot[i] += slot_value(row + i, bits[i]);
   }
+  // emptied after all reads: vmcnt counts stores too, so stores between the
+  // rows would make every row's loads wait for the previous row's stores
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kFusedBlock)
+#pragma unroll
+    for (int i = 0; i < kMaxAcc; ++i)
+      if (i < na) empty_slot(&partials[(uint64_t)b * na + i]);
   __syncthreads();  // red is reused
   const double t = block_total(tot, na, red);
   if (threadIdx.x < na) out[threadIdx.x] = t;
