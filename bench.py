@@ -8,11 +8,17 @@ storage).  `value` is the FP32 form's path-steps/s over the timed steps.
 
 Printed as one JSON line (rank 0, one GPU).  Extra keys carry the FP64 form,
 the per-strike error against the FP64 CPU oracle, Merton, LSMC timings, the
-roofline of the dominant kernel (timed with HIP events on the stream it runs
-on) and the CPU baseline (the C oracle with OpenMP on the host cores, on the
-same workload).
+roofline of the dominant kernel (its HIP-event duration averaged over the
+timed calls, on the stream it runs on) and the CPU baselines: the C oracle
+with OpenMP on the host cores, timed like examples/benchmark.py:24-31 (a
+warm-up, then min and median of 5), all cores on the full Heston workload
+and one core on a 1/16 sample, and LSMC on a 262,144-path sample plus one
+1,048,576-path run for the oracle's price.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
+
+Under rocprofv3, run with STOCHA_AMD_TORCH=0 so that the process maps one
+HIP runtime (the library's, the profiler's version) instead of torch's too.
 """
 from __future__ import annotations
 
@@ -22,6 +28,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -60,32 +68,110 @@ DRAWS_PER_HESTON_STEP = 2 * 1.0296
 # of them), the tables and the arguments.
 TRAFFIC_BYTES = int((219.8875 * 262144.0 / 131092.75 + 384.09375 * 1.0) * 1024)
 OPS_PER_STEP_HESTON32 = DRAWS_PER_HESTON_STEP * (25 + 5) + 14
-def _sync():
-    import torch
-    torch.cuda.synchronize()
+def time_calls(fn, steps, warmup, S=None):
+    """Wall time per call over `steps` calls after `warmup` untimed ones.
 
-
-def time_calls(fn, steps, warmup):
+    Every entry point synchronises its stream before it returns, so the wall
+    clock brackets finished device work without a separate sync.  With S
+    (the package) the HIP-event probe is on for the timed calls and the
+    second value is the dominant kernel's average duration over exactly
+    those calls (ms), measured on the stream the kernel runs on."""
     for _ in range(warmup):
         fn()
-    _sync()
+    if S is not None:
+        S.enable_kernel_timing(True)
+    kern = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
-    _sync()
-    return (time.perf_counter() - t0) / steps
+        if S is not None:
+            kern += S.last_kernel_ms()[0]
+    dt = (time.perf_counter() - t0) / steps
+    if S is not None:
+        S.enable_kernel_timing(False)
+    return dt, (kern / steps if S is not None else None)
 
 
-def kernel_ms(S, fn, reps=3):
-    """Average duration of the dominant kernel (HIP events on its stream)."""
-    S.enable_kernel_timing(True)
-    tot = 0.0
-    for _ in range(reps):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def min_median(fn, n_repeat=5):
+    """examples/benchmark.py:24-31: one warm-up, then n_repeat timed runs;
+    min and median wall time (s)."""
+    fn()
+    ts = []
+    for _ in range(n_repeat):
+        t0 = time.perf_counter()
         fn()
-        ms, _ = S.last_kernel_ms()
-        tot += ms
-    S.enable_kernel_timing(False)
-    return tot / reps
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[0], ts[len(ts) // 2]
+
+
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baselines(S, out, r32, r64):
+    """The C oracle (-O2, OpenMP over paths) on the host cores, timed like
+    examples/benchmark.py:24-31 (one warm-up, min and median of 5)."""
+    from oracle import oracle as O
+    O.build()
+    nproc = os.cpu_count() or 1
+    threads = min(nproc, 16)
+    p = list(HESTON.values())
+    ps = N_PATHS * STEPS
+    res = {}
+
+    def heston_cpu(n):
+        term = O.terminals(1, p, T, STEPS, n, SEED)
+        res["s1"], _ = O.call_sums(term, STRIKES)
+
+    O.set_threads(threads)
+    t_min, t_med = min_median(lambda: heston_cpu(N_PATHS))
+    ref = math.exp(-R * T) * res["s1"] / N_PATHS
+    # single thread on a 1/16 sample of the paths (same per-path work)
+    n1 = N_PATHS // 16
+    O.set_threads(1)
+    t1_min, t1_med = min_median(lambda: heston_cpu(n1))
+    O.set_threads(threads)
+    out["cpu_baseline"] = {
+        "value": ps / t_min, "unit": "path-steps/s", "cores": threads, "kind": "port",
+        "sample": f"full workload: {N_PATHS} Heston paths x {STEPS} steps, FP64 oracle, calls at "
+                  f"90/100/110; min of 5 after a warm-up",
+        "median_path_steps_per_s": ps / t_med, "seconds_min": t_min, "seconds_median": t_med,
+        "single_thread": {"path_steps_per_s_min": n1 * STEPS / t1_min,
+                          "path_steps_per_s_median": n1 * STEPS / t1_med,
+                          "sample": f"{n1} paths x {STEPS} steps (1/16 of the workload)"},
+        "nproc": nproc, "cpu_model": cpu_model()}
+    out["rel_err_vs_fp64_oracle"] = {
+        "fp32": (np.abs(r32["price"] - ref) / ref).tolist(),
+        "fp64": (np.abs(r64["price"] - ref) / ref).tolist(),
+        "oracle_price": ref.tolist()}
+    log(f"cpu heston: {ps / t_min:.3e} path-steps/s on {threads} threads")
+
+    # LSMC put (50 steps, degree 3): min/median of 5 on a 262,144-path
+    # sample, and one run at 1,048,576 paths for the oracle's price / SE
+    kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.20, t=1.0, steps=50, is_put=True,
+              poly_degree=3, seed=SEED)
+    ns = 262_144
+    l_min, l_med = min_median(lambda: O.lsmc(**kw, n_paths=ns))
+    t0 = time.perf_counter()
+    p1, se1 = O.lsmc(**kw, n_paths=N_PATHS)
+    l1 = time.perf_counter() - t0
+    out["cpu_baseline_lsmc"] = {
+        "ms_min": l_min * 1e3, "ms_median": l_med * 1e3, "cores": threads, "kind": "port",
+        "sample": f"{ns} paths x 50 steps (min/median of 5 after a warm-up)",
+        "ms_1048576_one_run": l1 * 1e3, "price_1048576": p1, "std_err_1048576": se1}
+    log(f"cpu lsmc: {l_min * 1e3:.1f} ms at {ns} paths, {l1 * 1e3:.0f} ms at {N_PATHS}")
 
 
 def main():
@@ -93,8 +179,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extras", action="store_true", help="skip Merton and LSMC")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one 256 MiB device-to-device copy (torch), the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
@@ -104,21 +190,20 @@ def main():
               "has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import stocha_amd as S
 
     def heston(prec):
         return S.european_price("heston", HESTON, T, STEPS, N_PATHS, STRIKES, r=R, seed=SEED,
                                 precision=prec)
 
-    # ---- timed: FP32 fused (the headline) ----
-    t32 = time_calls(lambda: heston(32), args.steps, args.warmup)
-    k32 = kernel_ms(S, lambda: heston(32))
+    # ---- timed: FP32 fused (the headline); kernel time over the same calls ----
+    t32, k32 = time_calls(lambda: heston(32), args.steps, args.warmup, S)
     r32 = heston(32)
+    log(f"heston fp32: {t32 * 1e3:.4f} ms/call, kernel {k32:.4f} ms")
     # ---- FP64 plain form (reference arithmetic), same paths ----
-    t64 = time_calls(lambda: heston(64), max(2, args.steps // 2), 1)
-    k64 = kernel_ms(S, lambda: heston(64))
+    t64, k64 = time_calls(lambda: heston(64), max(2, args.steps // 2), 1, S)
     r64 = heston(64)
+    log(f"heston fp64: {t64 * 1e3:.4f} ms/call, kernel {k64:.4f} ms")
 
     ps = N_PATHS * STEPS
     value = ps / t32
@@ -151,164 +236,13 @@ def main():
                        "traffic": TRAFFIC_BYTES,
                        "kernel": "terminal_feed_kernel<HestonSrc<fp32>, Heston32, CallPayoff>",
                        "ops_per_path_step": OPS_PER_STEP_HESTON32,
-                       "kernel_ms": k32}
+                       "kernel_ms": k32, "kernel_ms_over": f"the {args.steps} timed calls"}
 
-    # ---- CPU baseline: the C oracle (OpenMP) on the same workload ----
     if not args.no_cpu:
-        from oracle import oracle as O
-        O.build()
-        p = list(HESTON.values())
-        t0 = time.perf_counter()
-        term = O.terminals(1, p, T, STEPS, N_PATHS, SEED)
-        s1, _ = O.call_sums(term, STRIKES)
-        tc = time.perf_counter() - t0
-        ref = math.exp(-R * T) * s1 / N_PATHS
-        out["cpu_baseline"] = {"value": ps / tc, "unit": "path-steps/s",
-                               "cores": O.num_threads(), "kind": "port",
-                               "sample": f"full workload: {N_PATHS} Heston paths x {STEPS} "
-                                         "steps, FP64, calls at 90/100/110 (one run)",
-                               "seconds": tc}
-        out["rel_err_vs_fp64_oracle"] = {
-            "fp32": (np.abs(r32["price"] - ref) / ref).tolist(),
-            "fp64": (np.abs(r64["price"] - ref) / ref).tolist(),
-            "oracle_price": ref.tolist()}
+        cpu_baselines(S, out, r32, r64)
 
     if not args.no_extras:
-        def merton(prec):
-            return S.european_price("merton", MERTON, T, STEPS, N_PATHS, STRIKES, r=R,
-                                    seed=SEED, precision=prec)
-        m32 = time_calls(lambda: merton(32), 8, 2)
-        m64 = time_calls(lambda: merton(64), 6, 2)
-        out["merton"] = {"fp32_path_steps_per_s": ps / m32, "fp64_path_steps_per_s": ps / m64,
-                         "speedup": m64 / m32}
-        lsmc = {}
-        for n in (1_048_576, 4_194_304):
-            kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.20, t=1.0, steps=50, n_paths=n,
-                      is_put=True, poly_degree=3, seed=SEED)
-            tl = time_calls(lambda: S.lsmc_american_option_ex(**kw, precision=32), 2, 1)
-            price, se = S.lsmc_american_option_ex(**kw, precision=32)
-            tl64 = time_calls(lambda: S.lsmc_american_option(**kw), 2, 1)
-            p64, se64 = S.lsmc_american_option(**kw)
-            lsmc[str(n)] = {"fp32_ms": tl * 1e3, "fp32_price": price, "fp32_std_err": se,
-                            "fp64_ms": tl64 * 1e3, "fp64_price": p64, "fp64_std_err": se64}
-        out["lsmc_ms"] = lsmc
-
-        # GBM path matrix, 100,000 x 253 FP64 (gbm.rs); kernel time from the
-        # HIP-event probe, wall time including the copy to host memory
-        gkw = dict(s0=100.0, mu=0.05, sigma=0.2, t=1.0, steps=252, n_paths=100_000, seed=SEED)
-        tg = time_calls(lambda: S.gbm(**gkw), 2, 1)
-        kg = kernel_ms(S, lambda: S.gbm(**gkw))
-        gsteps = gkw["n_paths"] * gkw["steps"]
-        out["gbm_paths_100k"] = {"kernel_ms": kg, "wall_ms_incl_d2h": tg * 1e3,
-                                 "path_steps_per_s_kernel": gsteps / (kg * 1e-3),
-                                 "hbm_gbs_kernel": gkw["n_paths"] * 253 * 8 / (kg * 1e-3) / 1e9}
-
-        # bump-and-revalue Greeks, GBM call 1,048,576 x 252, all five Greeks:
-        # 8 bump scenarios in FP64 on shared draws (one launch), against the
-        # same 8 scenarios priced one launch each (FP64 fused European kernel)
-        fkw = dict(model="gbm", params={"s0": 100.0, "r": R, "sigma": 0.2, "t": T},
-                   payoff="call", strike=100.0, n_paths=N_PATHS, n_steps=STEPS,
-                   greeks=["delta", "gamma", "vega", "theta", "rho"], seed=SEED)
-        tf = time_calls(lambda: S.greeks_fd(**fkw), 3, 1)
-        kf = kernel_ms(S, lambda: S.greeks_fd(**fkw))
-
-        def one_per_launch():
-            for _ in range(8):
-                S.european_price("gbm", {"s0": 100.0, "mu": R, "sigma": 0.2}, T, STEPS, N_PATHS,
-                                 [100.0], r=R, seed=SEED, precision=64)
-        t1 = time_calls(one_per_launch, 2, 1)
-        out["greeks_fd_gbm_1Mx252"] = {
-            "ms": tf * 1e3, "kernel_ms": kf, "scenarios": 8,
-            "scenario_path_steps_per_s": 8 * N_PATHS * STEPS / tf,
-            "ms_one_launch_per_scenario": t1 * 1e3, "speedup": t1 / tf,
-            "values": S.greeks_fd(**fkw)}
-
-        # Heston QE (Andersen), 1,048,576 x 252 terminal prices, FP64: the
-        # variance recursion in the draw source, the log-price step per lane
-        from stocha_amd import _stocha as SS
-        qv = np.array([100.0, 0.04, 0.05, 2.0, 0.04, 0.3, -0.7, T])
-        tq = time_calls(lambda: SS._terminal_prices(1, qv, "qe", STEPS, N_PATHS, SEED), 3, 1)
-        kq = kernel_ms(S, lambda: SS._terminal_prices(1, qv, "qe", STEPS, N_PATHS, SEED))
-        te = time_calls(lambda: SS._terminal_prices(1, qv, "euler", STEPS, N_PATHS, SEED), 3, 1)
-        ke = kernel_ms(S, lambda: SS._terminal_prices(1, qv, "euler", STEPS, N_PATHS, SEED))
-        out["heston_qe_terminals_1Mx252"] = {
-            "ms": tq * 1e3, "kernel_ms": kq, "path_steps_per_s_kernel": ps / (kq * 1e-3),
-            "euler_same_api_ms": te * 1e3, "euler_kernel_ms": ke}
-
-        # path-dependent Monte Carlo on GBM, 1,048,576 x 252, FP64 (method="mc"):
-        # up-and-out call with a rebate at the hit, floating lookback call,
-        # arithmetic Asian call with the geometric control variate
-        ex = {}
-        ekw = dict(s=100.0, r=R, sigma=0.2, t=T, n_paths=N_PATHS, seed=SEED, method="mc")
-        cases = {
-            "barrier_uo_call_rebate": lambda: S.barrier_price(k=100.0, barrier=120.0,
-                                                              n_steps=STEPS, rebate=3.0,
-                                                              rebate_at_hit=True, **ekw),
-            "lookback_floating_call": lambda: S.lookback_price(n_steps=STEPS, **ekw),
-            "asian_arithmetic_call_cv": lambda: S.asian_price(k=100.0, n_steps=STEPS, **ekw),
-        }
-        for name, fn in cases.items():
-            te = time_calls(fn, 3, 1)
-            ke = kernel_ms(S, fn)
-            ex[name] = {"ms": te * 1e3, "kernel_ms": ke, "price": fn(),
-                        "path_steps_per_s": ps / te}
-        out["exotics_mc_1Mx252"] = ex
-
-        # RNG class bulk draws of one sequential stream, 100M values each
-        # (kernel time: the GPU work of the call; wall time adds the copy of
-        # 800 MB into pageable host memory)
-        rng_ex = {}
-        nbulk = 100_000_000
-        for name, fn in (("uniform", lambda: S.RNG(SEED).uniform(nbulk)),
-                         ("standard_normal", lambda: S.RNG(SEED).standard_normal(nbulk))):
-            tw = time_calls(fn, 2, 1)
-            kb = kernel_ms(S, fn, reps=2)
-            rng_ex[name] = {"values": nbulk, "wall_ms": tw * 1e3, "kernel_ms": kb,
-                            "values_per_s_kernel": nbulk / (kb * 1e-3),
-                            "hbm_gbs_written": nbulk * 8 / (kb * 1e-3) / 1e9}
-        out["rng_class_100M"] = rng_ex
-
-        # single-stream copulas, 1M samples of dimension 4 (kernel time: the
-        # GPU work; wall time adds the 32 MB copy to pageable host memory)
-        cop = {}
-        ccorr = np.array([[1.0, 0.5, 0.2, 0.1], [0.5, 1.0, 0.3, 0.2],
-                          [0.2, 0.3, 1.0, 0.4], [0.1, 0.2, 0.4, 1.0]])
-        ncop = 1_000_000
-        for name, fn in (("gaussian", lambda: S.gaussian_copula(ccorr, n_samples=ncop, seed=SEED)),
-                         ("student_t_nu5", lambda: S.student_t_copula(ccorr, nu=5.0,
-                                                                      n_samples=ncop,
-                                                                      seed=SEED))):
-            tw = time_calls(fn, 2, 1)
-            kb = kernel_ms(S, fn, reps=2)
-            cop[name] = {"samples": ncop, "dim": 4, "wall_ms": tw * 1e3, "kernel_ms": kb,
-                         "samples_per_s_kernel": ncop / (kb * 1e-3)}
-        out["copula_1Mx4"] = cop
-
-        # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
-        from stocha_amd import batch as B
-        inst = B.config4_instances(4096, seed=SEED)
-        plan = B.Plan(inst, seed=SEED)
-        tb = time_calls(plan.run, 2, 1)
-        out["batch_4096"] = {"ms": tb * 1e3, "path_steps": B.path_steps(inst),
-                             "path_steps_per_s": B.path_steps(inst) / tb, **plan.info()}
-        plan.close()
-
-        # basket call, d = 64 correlated GBM, 262,144 paths x 252 steps, FP32
-        # (L * sum z on MFMA) with the pathwise adjoint for all 64 assets
-        rng = np.random.default_rng(SEED)
-        a = rng.standard_normal((64, 64))
-        c = a @ a.T / 64 + 0.1 * np.eye(64)
-        dg = np.sqrt(np.diag(c))
-        corr = c / np.outer(dg, dg)
-        bs0 = rng.uniform(50, 200, 64)
-        bmu = rng.uniform(0.01, 0.08, 64)
-        bsig = rng.uniform(0.1, 0.4, 64)
-        w = np.full(64, 1.0 / 64)
-        bkw = dict(s0=bs0, mu=bmu, sigma=bsig, corr=corr, weights=w, strike=float(w @ bs0),
-                   r=R, t=T, steps=252, n_paths=262_144, seed=SEED, precision=32, greeks=True)
-        tk = time_calls(lambda: S.basket_price(**bkw), 2, 1)
-        out["basket_d64"] = {"ms": tk * 1e3, "normals_per_s": 262_144 * 252 * 64 / tk,
-                             "price": S.basket_price(**bkw)["price"]}
+        extras(S, out)
 
     if args.calib_copy:
         import torch
@@ -319,6 +253,160 @@ def main():
         out["calib_copy_bytes"] = 1 << 28
 
     print(json.dumps(out))
+
+
+def extras(S, out):
+    """Beside the headline: Merton, LSMC, the GBM path matrix, greeks_fd,
+    Heston QE, exotics, the RNG class, copulas, the batch and the basket."""
+    ps = N_PATHS * STEPS
+
+    def merton(prec):
+        return S.european_price("merton", MERTON, T, STEPS, N_PATHS, STRIKES, r=R,
+                                seed=SEED, precision=prec)
+    m32, _ = time_calls(lambda: merton(32), 8, 2)
+    m64, _ = time_calls(lambda: merton(64), 6, 2)
+    out["merton"] = {"fp32_path_steps_per_s": ps / m32, "fp64_path_steps_per_s": ps / m64,
+                     "speedup": m64 / m32}
+    log("merton done")
+
+    # LSMC put, 50 steps, degree 3: wall time per call and the two kernels'
+    # HIP-event time (path generation + cooperative induction)
+    lsmc = {}
+    for n in (1_048_576, 4_194_304):
+        kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.20, t=1.0, steps=50, n_paths=n,
+                  is_put=True, poly_degree=3, seed=SEED)
+        tl, kl = time_calls(lambda: S.lsmc_american_option_ex(**kw, precision=32), 5, 1, S)
+        price, se = S.lsmc_american_option_ex(**kw, precision=32)
+        tl64, kl64 = time_calls(lambda: S.lsmc_american_option(**kw), 5, 1, S)
+        p64, se64 = S.lsmc_american_option(**kw)
+        # algorithmic HBM bytes: the FP32 path store written once (n x 50 x 4 B)
+        # and read once by the induction (n x 49 x 4 B)
+        hbm = n * (50 + 49) * 4
+        lsmc[str(n)] = {"fp32_ms": tl * 1e3, "fp32_kernels_ms": kl, "fp32_price": price,
+                        "fp32_std_err": se, "fp64_ms": tl64 * 1e3, "fp64_kernels_ms": kl64,
+                        "fp64_price": p64, "fp64_std_err": se64,
+                        "roofline_fp32": {"bound": "hbm", "achieved": hbm / (kl * 1e-3) / 1e9,
+                                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                          "frac": hbm / (kl * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                          "bytes": hbm}}
+    out["lsmc_ms"] = lsmc
+    log("lsmc done")
+
+    # GBM path matrix, 100,000 x 253 FP64 (gbm.rs): kernel time from the
+    # HIP-event probe, wall time including the copy to host memory
+    gkw = dict(s0=100.0, mu=0.05, sigma=0.2, t=1.0, steps=252, n_paths=100_000, seed=SEED)
+    tg, kg = time_calls(lambda: S.gbm(**gkw), 5, 1, S)
+    gsteps = gkw["n_paths"] * gkw["steps"]
+    gbytes = gkw["n_paths"] * 253 * 8
+    out["gbm_paths_100k"] = {"kernel_ms": kg, "wall_ms_incl_d2h": tg * 1e3,
+                             "path_steps_per_s_kernel": gsteps / (kg * 1e-3),
+                             "path_steps_per_s_wall": gsteps / tg,
+                             "roofline": {"bound": "hbm",
+                                          "achieved": gbytes / (kg * 1e-3) / 1e9,
+                                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                          "frac": gbytes / (kg * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                          "bytes": gbytes}}
+    log("gbm paths done")
+
+    # bump-and-revalue Greeks, GBM call 1,048,576 x 252, all five Greeks:
+    # 8 bump scenarios in FP64 on shared draws (one launch), against the
+    # same 8 scenarios priced one launch each (FP64 fused European kernel)
+    fkw = dict(model="gbm", params={"s0": 100.0, "r": R, "sigma": 0.2, "t": T},
+               payoff="call", strike=100.0, n_paths=N_PATHS, n_steps=STEPS,
+               greeks=["delta", "gamma", "vega", "theta", "rho"], seed=SEED)
+    tf, kf = time_calls(lambda: S.greeks_fd(**fkw), 3, 1, S)
+
+    def one_per_launch():
+        for _ in range(8):
+            S.european_price("gbm", {"s0": 100.0, "mu": R, "sigma": 0.2}, T, STEPS, N_PATHS,
+                             [100.0], r=R, seed=SEED, precision=64)
+    t1, _ = time_calls(one_per_launch, 2, 1)
+    out["greeks_fd_gbm_1Mx252"] = {
+        "ms": tf * 1e3, "kernel_ms": kf, "scenarios": 8,
+        "scenario_path_steps_per_s": 8 * N_PATHS * STEPS / tf,
+        "ms_one_launch_per_scenario": t1 * 1e3, "speedup": t1 / tf,
+        "values": S.greeks_fd(**fkw)}
+
+    # Heston QE (Andersen), 1,048,576 x 252 terminal prices, FP64
+    from stocha_amd import _stocha as SS
+    qv = np.array([100.0, 0.04, 0.05, 2.0, 0.04, 0.3, -0.7, T])
+    tq, kq = time_calls(lambda: SS._terminal_prices(1, qv, "qe", STEPS, N_PATHS, SEED), 3, 1, S)
+    te, ke = time_calls(lambda: SS._terminal_prices(1, qv, "euler", STEPS, N_PATHS, SEED), 3, 1,
+                        S)
+    out["heston_qe_terminals_1Mx252"] = {
+        "ms": tq * 1e3, "kernel_ms": kq, "path_steps_per_s_kernel": ps / (kq * 1e-3),
+        "euler_same_api_ms": te * 1e3, "euler_kernel_ms": ke}
+    log("greeks_fd / qe done")
+
+    # path-dependent Monte Carlo on GBM, 1,048,576 x 252, FP64 (method="mc")
+    ex = {}
+    ekw = dict(s=100.0, r=R, sigma=0.2, t=T, n_paths=N_PATHS, seed=SEED, method="mc")
+    cases = {
+        "barrier_uo_call_rebate": lambda: S.barrier_price(k=100.0, barrier=120.0,
+                                                          n_steps=STEPS, rebate=3.0,
+                                                          rebate_at_hit=True, **ekw),
+        "lookback_floating_call": lambda: S.lookback_price(n_steps=STEPS, **ekw),
+        "asian_arithmetic_call_cv": lambda: S.asian_price(k=100.0, n_steps=STEPS, **ekw),
+    }
+    for name, fn in cases.items():
+        te, ke = time_calls(fn, 3, 1, S)
+        ex[name] = {"ms": te * 1e3, "kernel_ms": ke, "price": fn(),
+                    "path_steps_per_s": ps / te}
+    out["exotics_mc_1Mx252"] = ex
+
+    # RNG class bulk draws of one sequential stream, 100M values each
+    rng_ex = {}
+    nbulk = 100_000_000
+    for name, fn in (("uniform", lambda: S.RNG(SEED).uniform(nbulk)),
+                     ("standard_normal", lambda: S.RNG(SEED).standard_normal(nbulk))):
+        tw, kb = time_calls(fn, 2, 1, S)
+        rng_ex[name] = {"values": nbulk, "wall_ms": tw * 1e3, "kernel_ms": kb,
+                        "values_per_s_kernel": nbulk / (kb * 1e-3),
+                        "hbm_gbs_written": nbulk * 8 / (kb * 1e-3) / 1e9}
+    out["rng_class_100M"] = rng_ex
+
+    # single-stream copulas, 1M samples of dimension 4
+    cop = {}
+    ccorr = np.array([[1.0, 0.5, 0.2, 0.1], [0.5, 1.0, 0.3, 0.2],
+                      [0.2, 0.3, 1.0, 0.4], [0.1, 0.2, 0.4, 1.0]])
+    ncop = 1_000_000
+    for name, fn in (("gaussian", lambda: S.gaussian_copula(ccorr, n_samples=ncop, seed=SEED)),
+                     ("student_t_nu5", lambda: S.student_t_copula(ccorr, nu=5.0,
+                                                                  n_samples=ncop,
+                                                                  seed=SEED))):
+        tw, kb = time_calls(fn, 2, 1, S)
+        cop[name] = {"samples": ncop, "dim": 4, "wall_ms": tw * 1e3, "kernel_ms": kb,
+                     "samples_per_s_kernel": ncop / (kb * 1e-3)}
+    out["copula_1Mx4"] = cop
+    log("exotics / rng / copulas done")
+
+    # batched launch: 4096 uneven GBM / Heston / Merton instances (FP32)
+    from stocha_amd import batch as B
+    inst = B.config4_instances(4096, seed=SEED)
+    plan = B.Plan(inst, seed=SEED)
+    tb, kb = time_calls(plan.run, 3, 1, S)
+    out["batch_4096"] = {"ms": tb * 1e3, "kernel_ms": kb, "path_steps": B.path_steps(inst),
+                         "path_steps_per_s": B.path_steps(inst) / tb, **plan.info()}
+    plan.close()
+
+    # basket call, d = 64 correlated GBM, 262,144 paths x 252 steps, FP32
+    # (L * sum z on MFMA) with the pathwise adjoint for all 64 assets
+    rng = np.random.default_rng(SEED)
+    a = rng.standard_normal((64, 64))
+    c = a @ a.T / 64 + 0.1 * np.eye(64)
+    dg = np.sqrt(np.diag(c))
+    corr = c / np.outer(dg, dg)
+    bs0 = rng.uniform(50, 200, 64)
+    bmu = rng.uniform(0.01, 0.08, 64)
+    bsig = rng.uniform(0.1, 0.4, 64)
+    w = np.full(64, 1.0 / 64)
+    bkw = dict(s0=bs0, mu=bmu, sigma=bsig, corr=corr, weights=w, strike=float(w @ bs0),
+               r=R, t=T, steps=252, n_paths=262_144, seed=SEED, precision=32, greeks=True)
+    tk, kk = time_calls(lambda: S.basket_price(**bkw), 3, 1, S)
+    out["basket_d64"] = {"ms": tk * 1e3, "kernels_ms": kk,
+                         "normals_per_s": 262_144 * 252 * 64 / tk,
+                         "price": S.basket_price(**bkw)["price"]}
+    log("batch / basket done")
 
 
 if __name__ == "__main__":

@@ -87,6 +87,8 @@ def lib():
                                            C.POINTER(C.c_uint64)]
         L.orc_student_t_copula.restype = None
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
         _lib = L
     return _lib
 
@@ -385,3 +387,7 @@ def student_t_copula(corr, nu, n_samples, seed=42, details=False):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))

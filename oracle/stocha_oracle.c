@@ -154,6 +154,16 @@ void orc_normals(uint64_t seed, uint64_t block, uint64_t path, uint64_t n, doubl
     }
 }
 
+/* OpenMP threads of the following calls (the CPU baseline's single-thread
+ * and all-core runs); n <= 0 keeps the current setting. */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -136,6 +136,7 @@ void orc_student_t_copula(const double *L, uint64_t d, double nu, uint64_t n_sam
                           uint64_t *used);
 
 /* Number of OpenMP threads the oracle will use. */
+void orc_set_threads(int n);
 int orc_num_threads(void);
 
 #ifdef __cplusplus

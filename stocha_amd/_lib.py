@@ -5,6 +5,10 @@ own HIP runtime, and the soname both share (libamdhip64.so.7) then resolves
 to that one copy, so device pointers from torch tensors and from this library
 live in the same HIP context.  Without torch the system ROCm runtime is used.
 
+STOCHA_AMD_TORCH=0 skips torch: the library then binds the ROCm runtime it
+was linked against (its RUNPATH), the only HIP runtime in the process --
+what a profiler of that ROCm version expects.
+
 There is no fallback: if the library is missing the import fails with an
 error that says how to build it.
 """
@@ -13,10 +17,12 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-try:  # share torch's HIP runtime when it is present (see module docstring)
-    import torch  # noqa: F401
-except Exception:  # pragma: no cover - torch is optional for the C ABI
-    torch = None
+torch = None
+if os.environ.get("STOCHA_AMD_TORCH", "1") != "0":
+    try:  # share torch's HIP runtime when it is present (see module docstring)
+        import torch  # noqa: F401
+    except Exception:  # pragma: no cover - torch is optional for the C ABI
+        torch = None
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libstocha_gpu.so")

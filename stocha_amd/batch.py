@@ -155,6 +155,8 @@ def _signature(instances):
 
 
 def _device_key():
+    if os.environ.get("STOCHA_AMD_TORCH", "1") == "0":
+        return "hip-device"  # keep torch's HIP runtime out of the process
     try:
         import torch
         return torch.cuda.get_device_name(0)
