@@ -25,7 +25,7 @@ if os.environ.get("STOCHA_AMD_TORCH", "1") != "0":
         torch = None
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libstocha_gpu.so")
+LIB_PATH = os.environ.get("STOCHA_AMD_LIB") or os.path.join(_HERE, "libstocha_gpu.so")
 
 STOCHA_OK, STOCHA_EINVAL, STOCHA_EDEVICE, STOCHA_ENOMEM, STOCHA_ENOTIMPL = 0, 1, 2, 3, 4
 STOCHA_HOST, STOCHA_DEVICE = 0, 1

@@ -10,28 +10,31 @@
 
 namespace stocha {
 
-// Step-input sources for queue_path.  A source turns the lane's draws, one
-// per call, into queue entries; Src::kPerStep entries make one step's input.
-// feed(u, z, rec, active) consumes u only when `active` (state unchanged
-// otherwise) and returns true when rec holds a new entry.
+// Step-input sources for queue_path.  A source turns the lanes' draws, one
+// per lane per call, into queue entries; Src::kPerStep entries make one
+// step's input.  Sources are wave-collective: feed(u, z, rec, act) is called
+// by every lane of the wavefront together, consumes u on the lanes of the
+// lane mask `act` (the others keep their state), and returns the lane mask of
+// lanes whose rec holds a new entry.  pending_mask(): the lanes whose next
+// draw goes to the ziggurat's slow block.
 //
 // GBM (gbm.rs:60-66) and Heston Euler (heston.rs:59-71) take only normals, so the
 // entries are single normals: 1 per GBM step, 2 per Heston step (dw1 first,
 // then the independent dz).
 template <bool kFp32, uint32_t kPer>
 struct NormalSrc {
-  using Real = typename ZigFeed<kFp32>::Real;
-  using Tables = typename ZigFeed<kFp32>::Tables;
+  using Real = typename ZigWave<kFp32>::Real;
+  using Tables = typename ZigWave<kFp32>::Tables;
   using Rec = Real;
   static constexpr uint32_t kPerStep = kPer;
   static constexpr uint32_t kDefer = 8;  // slow-block deferral (queue_run)
-  static constexpr int kWaves = kFp32 ? 5 : 4;  // occupancy target of the fused kernel
-  ZigFeed<kFp32> zf;
+  static constexpr int kWaves = kFp32 ? 6 : 4;  // occupancy target of the fused kernel
+  ZigWave<kFp32> zf;
   template <class C>
   __device__ __forceinline__ explicit NormalSrc(const C&) {}
-  __device__ __forceinline__ bool pending() const { return zf.pending(); }
-  __device__ __forceinline__ bool feed(uint64_t u, const Tables* z, Rec& r, bool active) {
-    return zf.feed(u, z, r, active);
+  __device__ __forceinline__ uint64_t pending_mask() const { return zf.pending_mask(); }
+  __device__ __forceinline__ uint64_t feed(uint64_t u, const Tables* z, Rec& r, uint64_t act) {
+    return zf.feed(u, z, r, act);
   }
   template <uint32_t kSlots, int kCols, class M>
   __device__ static __forceinline__ void apply(M& m, const Rec (*q)[kCols], uint32_t head) {
@@ -52,8 +55,8 @@ using HestonSrc = NormalSrc<kFp32, 2>;
 // jump return (0 without a jump).
 template <bool kFp32, class M>
 struct MertonSrc {
-  using Real = typename ZigFeed<kFp32>::Real;
-  using Tables = typename ZigFeed<kFp32>::Tables;
+  using Real = typename ZigWave<kFp32>::Real;
+  using Tables = typename ZigWave<kFp32>::Tables;
   struct Rec {
     Real z, jr;
   };
@@ -62,29 +65,29 @@ struct MertonSrc {
   // and the uniform, waiting lanes cost more draws than the shared pass saves
   static constexpr uint32_t kDefer = 0;
   static constexpr int kWaves = 4;
-  ZigFeed<kFp32> zf;
-  uint32_t phase = 0;  // 0: z next, 1: uniform next, 2: jump normal next
+  ZigWave<kFp32> zf;
+  uint64_t uni_next = 0;   // lanes whose next draw is the uniform
+  uint64_t jump_next = 0;  // lanes whose next normal is the jump size
   Real zz = 0;
   uint64_t t53;  // next_f64() < lambda*dt  <=>  (u >> 11) < t53
   M jm;          // for jump_return()
   template <class C>
   __device__ __forceinline__ explicit MertonSrc(const C& c) : t53(c.jump_t53), jm(c, false) {}
-  // (phase 1 is entered with the ziggurat idle, so only its states count)
-  __device__ __forceinline__ bool pending() const { return zf.pending(); }
-  // Straight-line: the phase transitions and the entry are selects; only the
-  // ziggurat's own slow block branches.  rec is written on every call.
-  __device__ __forceinline__ bool feed(uint64_t u, const Tables* z, Rec& r, bool active) {
+  __device__ __forceinline__ uint64_t pending_mask() const { return zf.pending_mask(); }
+  // The phase transitions are mask operations; rec is written on every lane.
+  __device__ __forceinline__ uint64_t feed(uint64_t u, const Tables* z, Rec& r, uint64_t act) {
     Real x;
-    const bool uni = active & (phase == 1u);
-    const bool got = zf.feed(u, z, x, active & !uni);
-    const bool jump = (u >> 11) < t53;  // the uniform draw (pcg64.rs:37-39)
-    const bool got0 = got & (phase == 0u);
-    const bool got2 = got & (phase == 2u);
+    const uint64_t uni = act & uni_next;
+    const uint64_t got = zf.feed(u, z, x, act & ~uni);
+    const uint64_t jump = lanes((u >> 11) < t53);  // the uniform draw (pcg64.rs:37-39)
+    const uint64_t got0 = got & ~jump_next;  // the diffusion normal z
+    const uint64_t got2 = got & jump_next;   // the jump normal
     r.z = zz;
-    r.jr = got2 ? jm.jump_return(x) : (Real)0;
-    zz = got0 ? x : zz;
-    phase = uni ? (jump ? 2u : 0u) : (got0 ? 1u : (got2 ? 0u : phase));
-    return (uni & !jump) | got2;
+    r.jr = lane_in(got2) ? jm.jump_return(x) : (Real)0;
+    zz = lane_in(got0) ? x : zz;
+    uni_next = (uni_next & ~uni) | got0;
+    jump_next = (jump_next & ~got2) | (uni & jump);
+    return (uni & ~jump) | got2;
   }
   template <uint32_t kSlots, int kCols, class MM>
   __device__ static __forceinline__ void apply(MM& m, const Rec (*q)[kCols], uint32_t head) {
@@ -110,7 +113,7 @@ struct HestonQeSrc {
   static constexpr uint32_t kPerStep = 1;
   static constexpr uint32_t kDefer = 0;
   static constexpr int kWaves = 4;
-  ZigFeed<false> zf;
+  ZigFeed<false> zf;  // the per-lane sampler (its slow path is the rarer one here)
   uint32_t phase = 0;  // 0: variance draw next, 1: asset normal next
   bool quad = true;    // the pending variance draw is a normal (quadratic branch)
   double qa = 0, qb = 0;  // quadratic: a, b; exponential: p, beta
@@ -140,8 +143,11 @@ struct HestonQeSrc {
       qb = (1.0 - p) / m;
     }
   }
-  __device__ __forceinline__ bool pending() const { return zf.pending(); }
-  __device__ __forceinline__ bool feed(uint64_t u, const Tables* z, Rec& r, bool active) {
+  __device__ __forceinline__ uint64_t pending_mask() const { return lanes(zf.pending()); }
+  __device__ __forceinline__ uint64_t feed(uint64_t u, const Tables* z, Rec& r, uint64_t act) {
+    return lanes(feed_lane(u, z, r, lane_in(act)));
+  }
+  __device__ __forceinline__ bool feed_lane(uint64_t u, const Tables* z, Rec& r, bool active) {
     double x;
     const bool uni = active & (phase == 0u) & !quad;
     const bool got = zf.feed(u, z, x, active & !uni);
@@ -211,6 +217,8 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
   }
   uint32_t head = 0, cnt = 0;
   for (uint32_t step = 0; step < steps; ++step) {
+    // lanes that still lack this step's input (a wave-uniform lane mask)
+    uint64_t need = lanes(cnt < kNeed);
     // bottom-tested: one draw per step is always taken (by lanes with room),
     // which keeps the loop-carried state in place across the test
     do {
@@ -219,23 +227,26 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
       // dry, so that one pass through the slow block serves many lanes
       // (without the wait, 69% of a wave's draws would take it).  The lane's
       // own draws are consumed in the same order either way.
-      bool active = cnt < kCap;
+      uint64_t act = lanes(cnt < kCap);
       if constexpr (Src::kDefer > 0) {
-        const bool pend = src.pending();
-        const uint64_t waiting = __builtin_amdgcn_ballot_w64(pend);
-        const uint64_t dry = waiting & __builtin_amdgcn_ballot_w64(cnt < kNeed);
-        const bool go = ((uint32_t)__builtin_popcountll(waiting) >= Src::kDefer) | (dry != 0);
-        active = active & (!pend | go);
+        const uint64_t waiting = src.pending_mask();
+        // 32-bit popcounts: the scalar unit has no 64-bit ordered compare
+        const uint32_t n_wait = (uint32_t)__builtin_popcount((uint32_t)waiting) +
+                                (uint32_t)__builtin_popcount((uint32_t)(waiting >> 32));
+        const bool go = (n_wait >= Src::kDefer) | ((waiting & need) != 0);
+        act &= go ? ~0ull : ~waiting;
       }
       const uint64_t u = g.output();
       const U128 sn = g.stepped(incv);
-      g.s.lo = active ? sn.lo : g.s.lo;
-      g.s.hi = active ? sn.hi : g.s.hi;
+      const bool a = lane_in(act);
+      g.s.lo = a ? sn.lo : g.s.lo;
+      g.s.hi = a ? sn.hi : g.s.hi;
       Rec r;
-      const bool got = src.feed(u, zig, r, active);
+      const uint64_t got = src.feed(u, zig, r, act);
       q[(head + cnt) & (kSlots - 1)][threadIdx.x] = r;
-      cnt += got ? 1u : 0u;
-    } while (__any(cnt < kNeed));
+      cnt += lane_in(got) ? 1u : 0u;
+      need = lanes(cnt < kNeed);
+    } while (need != 0);
     Src::template apply<kSlots, kCols>(m, q, head);
     head = (head + kNeed) & (kSlots - 1);
     cnt -= kNeed;

@@ -30,18 +30,21 @@ constexpr int kMaxStrikes = 8;
 static_assert(kMaxAcc >= 2 * kMaxStrikes, "accumulators");
 constexpr int kMaxScen = 8;  // bump scenarios per launch (greeks_fd)
 
+// Payoffs write one path's contributions to the sums: v[0..n_acc()), zero for
+// a lane without a path (valid false).
 struct CallPayoff {
   double k[kMaxStrikes];
   int n;
   __device__ __forceinline__ int n_acc() const { return 2 * n; }
-  __device__ __forceinline__ void add(double st, double* acc) const {
+  __device__ __forceinline__ void terms(double st, bool valid, double* v) const {
 #pragma unroll
     for (int i = 0; i < kMaxStrikes; ++i) {
       if (i < n) {
         double pay = st - k[i];
         pay = pay > 0.0 ? pay : 0.0;
-        acc[2 * i] += pay;
-        acc[2 * i + 1] += pay * pay;
+        pay = valid ? pay : 0.0;
+        v[2 * i] = pay;
+        v[2 * i + 1] = pay * pay;
       }
     }
   }
@@ -52,13 +55,15 @@ struct PathwisePayoff {
   double s0, strike, drift_t, sigma;  // drift_t = (r + 0.5*sigma*sigma)*t
   int is_call;
   __device__ __forceinline__ int n_acc() const { return 2; }
-  __device__ __forceinline__ void add(double s_t, double* acc) const {
-    const bool in_money = is_call ? s_t > strike : s_t < strike;
+  __device__ __forceinline__ void terms(double s_t, bool valid, double* v) const {
+    v[0] = 0.0;
+    v[1] = 0.0;
+    const bool in_money = valid && (is_call ? s_t > strike : s_t < strike);
     if (in_money) {
       const double sign = is_call ? 1.0 : -1.0;
-      acc[0] += sign * s_t / s0;
+      v[0] = sign * s_t / s0;
       const double log_return = log(s_t / s0);
-      acc[1] += sign * s_t * (log_return - drift_t) / sigma;
+      v[1] = sign * s_t * (log_return - drift_t) / sigma;
     }
   }
 };
@@ -79,26 +84,33 @@ __device__ __forceinline__ double terminal_value(const Const& mc, const StreamDe
   }
 }
 
+// Both schedules accumulate the same way: per round of paths (one path per
+// lane), the payoff terms of the wave go into the wave's LDS row
+// (wave_accumulate), so the loop and feed kernels sum in the same order and
+// give the same bits.
 template <class Model, class Const, bool kFp32, class Payoff>
 __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
     const Const mc, const StreamDesc sd, const uint64_t n_paths, const uint64_t steps,
     const Payoff pay, double* __restrict__ partials, unsigned* __restrict__ counter,
     double* __restrict__ out) {
   __shared__ ZigLds zig;
-  __shared__ double red[kFusedBlock / 64][kMaxAcc];
+  __shared__ double wacc[kFusedBlock / 64][kMaxAcc];
   zig_load(&zig);
+  if (threadIdx.x < kFusedBlock / 64 * kMaxAcc) (&wacc[0][0])[threadIdx.x] = 0.0;
   __syncthreads();
 
-  double acc[kMaxAcc];
-#pragma unroll
-  for (int i = 0; i < kMaxAcc; ++i) acc[i] = 0.0;
-
   const uint64_t stride = (uint64_t)gridDim.x * kFusedBlock;
-  for (uint64_t p = (uint64_t)blockIdx.x * kFusedBlock + threadIdx.x; p < n_paths; p += stride) {
-    const double st = terminal_value<Model, Const, kFp32>(mc, sd, p, steps, &zig);
-    pay.add(st, acc);
+  for (uint64_t base = (uint64_t)blockIdx.x * kFusedBlock; base < n_paths; base += stride) {
+    const uint64_t p = base + threadIdx.x;
+    const bool valid = p < n_paths;
+    double st = 0.0;
+    if (valid) st = terminal_value<Model, Const, kFp32>(mc, sd, p, steps, &zig);
+    double v[kMaxAcc];
+    pay.terms(st, valid, v);
+    wave_accumulate(v, pay.n_acc(), wacc);
   }
-  finish_sums(acc, pay.n_acc(), red, partials, counter, out);
+  finish_block_sums(wave_rows_total(pay.n_acc(), wacc), pay.n_acc(), wacc, partials, counter,
+                    out);
 }
 
 // Queue kernel (the FEED schedule).  Each lane feeds its own draws into its
@@ -110,7 +122,7 @@ __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
 // the draw-count distribution of the ziggurat, a ring of 8 Heston normals
 // costs ~2.14 draws per step against the 2.06 minimum.
 // Occupancy target per source (Src::kWaves): the FP32 GBM/Heston kernels fit
-// 5 waves per SIMD (<= 96 VGPRs, 29 KB of LDS per block) without spilling;
+// 6 waves per SIMD (<= 80 VGPRs, 23 KB of LDS per block) without spilling;
 // the others need more registers and run 4.
 template <class Src, class Model, class Const, class Payoff, int kQueue>
 __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
@@ -121,13 +133,10 @@ __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
   using Rec = typename Src::Rec;
   __shared__ typename Src::Tables zig;
   __shared__ Rec q[kQueue][kFusedBlock];
-  __shared__ double red[kFusedBlock / 64][kMaxAcc];
+  __shared__ double wacc[kFusedBlock / 64][kMaxAcc];
   zig_load(&zig);
+  if (threadIdx.x < kFusedBlock / 64 * kMaxAcc) (&wacc[0][0])[threadIdx.x] = 0.0;
   __syncthreads();
-
-  double acc[kMaxAcc];
-#pragma unroll
-  for (int i = 0; i < kMaxAcc; ++i) acc[i] = 0.0;
 
   const U128 inc = sd.inc;
   const uint64_t stride = (uint64_t)gridDim.x * kFusedBlock;
@@ -136,9 +145,12 @@ __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
     const bool valid = p < n_paths;
     Pcg g{stream_at(&sd, valid ? p : 0)};
     const double st = queue_path<Src, Model, kQueue, kFusedBlock>(mc, g, inc, steps, &zig, q);
-    if (valid) pay.add(st, acc);
+    double v[kMaxAcc];
+    pay.terms(st, valid, v);
+    wave_accumulate(v, pay.n_acc(), wacc);
   }
-  finish_sums(acc, pay.n_acc(), red, partials, counter, out);
+  finish_block_sums(wave_rows_total(pay.n_acc(), wacc), pay.n_acc(), wacc, partials, counter,
+                    out);
 }
 
 // Sums partials[b][i] over b: block i owns accumulator i; thread t adds

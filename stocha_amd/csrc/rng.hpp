@@ -184,17 +184,21 @@ struct ZigFast32 {
   float xs;     // +-X[l] * 2^-32, rounded to FP32
 };
 
-// What the FP32 samplers read (12 KB) ...
+// What the FP32 queue samplers read per draw (6 KB): the fast path and the
+// FP32 wedge.  The exact threshold T and the FP64 X and Y, read only on a tied
+// fast test or inside the FP32 wedge's guard band (a few draws in a million),
+// come from constant memory (kZigT, kZigX, kZigY), which keeps the block's LDS
+// small enough for six resident blocks per CU.
 struct ZigLds32 {
   ZigFast32 fast32[512];
-  uint64_t t[256];    // T[l], exact fast-path threshold on m = u >> 11
   float2 ydy32[256];  // FP32 wedge: (Y[l], Y[l+1] - Y[l])
-  double x[257];
-  double y[257];
 };
-// ... and the FP64 fast-path table on top (20 KB in all).
+// ... and what the FP64 samplers read on top (12 KB): the FP64 fast path, T
+// for the loop-form fast test and Y for the FP64 wedge.
 struct ZigLds : ZigLds32 {
   ZigFast fast[512];
+  uint64_t t[256];  // T[l], exact fast-path threshold on m = u >> 11
+  double y[257];
 };
 
 __constant__ const double kZigX[257] = {ZIG_X_VALUES};
@@ -210,14 +214,7 @@ __device__ __forceinline__ void zig_load(ZigLds32* z) {
     const uint32_t th = (uint32_t)(kZigT[l] >> 21);
     z->fast32[i].th = th;
     z->fast32[i].xs = (float)(xv * 0x1p-32);
-    if (i < 256) {
-      z->t[i] = kZigT[i];
-      z->ydy32[i] = make_float2((float)kZigY[i], (float)(kZigY[i + 1] - kZigY[i]));
-    }
-    if (i < 257) {
-      z->x[i] = kZigX[i];
-      z->y[i] = kZigY[i];
-    }
+    if (i < 256) z->ydy32[i] = make_float2((float)kZigY[i], (float)(kZigY[i + 1] - kZigY[i]));
   }
 }
 __device__ __forceinline__ void zig_load(ZigLds* z) {
@@ -227,6 +224,8 @@ __device__ __forceinline__ void zig_load(ZigLds* z) {
     z->fast[i].th = (uint32_t)(kZigT[l] >> 21);
     z->fast[i].pad = 0u;
     z->fast[i].x = (i & 256) ? -kZigX[l] : kZigX[l];
+    if (i < 256) z->t[i] = kZigT[i];
+    if (i < 257) z->y[i] = kZigY[i];
   }
 }
 
@@ -339,13 +338,13 @@ __device__ __forceinline__ float zig_normal32(Pcg& g, const U128 inc, const ZigL
     const uint32_t idx = (uint32_t)u & 0x1ffu, layer = idx & 0xffu;
     const uint64_t m = u >> 11;
     const float xf = hi32_to_f32(u) * z->fast32[idx].xs;  // signed
-    if (m < z->t[layer]) return xf;
+    if (m < kZigT[layer]) return xf;
     if (layer == 0) {
       const float tl = zig_tail<float>(g, inc);
       return (idx & 256u) ? -tl : tl;
     }
-    const double x = ((double)m * 0x1p-53) * z->x[layer];
-    const double y = z->y[layer] + g.next_f64(inc) * (z->y[layer + 1] - z->y[layer]);
+    const double x = ((double)m * 0x1p-53) * kZigX[layer];
+    const double y = kZigY[layer] + g.next_f64(inc) * (kZigY[layer + 1] - kZigY[layer]);
     const float arg = (float)(-0.5 * x * x);
     const float e = __builtin_amdgcn_exp2f(arg * 1.44269504088896341f);
     const double diff = y - (double)e;

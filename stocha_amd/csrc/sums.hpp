@@ -81,6 +81,37 @@ __device__ __forceinline__ void empty_slot(double* p) {
                      __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Per-path accumulation without per-thread accumulators: after each path the
+// wave adds its lanes' contributions (a butterfly) into its own row of wacc
+// (LDS, kFusedBlock / 64 rows of kMaxAcc sums, zeroed by the caller), so no
+// FP64 sums stay live in registers across the path loop.  v[0..na): this
+// lane's contributions (0 for a lane without a path).
+__device__ __forceinline__ void wave_accumulate(const double* v, int na,
+                                                double (*wacc)[kMaxAcc]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMaxAcc; ++i) {
+    if (i < na) {
+      const double w = wave_sum(v[i]);
+      if (lane == 0) wacc[wave][i] += w;
+    }
+  }
+}
+
+// Block total of the wave rows (wave order): thread i < na gets sum i.
+__device__ __forceinline__ double wave_rows_total(int na, double (*wacc)[kMaxAcc]) {
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x < na)
+    for (int w = 0; w < kFusedBlock / 64; ++w) s += wacc[w][threadIdx.x];
+  return s;
+}
+
+__device__ __forceinline__ void finish_block_sums(double s, int na, double (*red)[kMaxAcc],
+                                                  double* partials,
+                                                  unsigned* __restrict__ counter,
+                                                  double* __restrict__ out);
+
 // End of a fused kernel: store the block partial; the last block to arrive
 // reduces all partials in a fixed order into out[0..na) and re-zeroes the
 // counter (every block has incremented it by then).  partials: slots that
@@ -92,8 +123,16 @@ __device__ __forceinline__ void empty_slot(double* p) {
 __device__ __forceinline__ void finish_sums(const double* acc, int na, double (*red)[kMaxAcc],
                                             double* partials, unsigned* __restrict__ counter,
                                             double* __restrict__ out) {
+  finish_block_sums(block_total(acc, na, red), na, red, partials, counter, out);
+}
+
+// finish_sums from the block totals (thread i < na holds sum i); red is
+// scratch LDS of the same shape as block_total's.
+__device__ __forceinline__ void finish_block_sums(double s, int na, double (*red)[kMaxAcc],
+                                                  double* partials,
+                                                  unsigned* __restrict__ counter,
+                                                  double* __restrict__ out) {
   __shared__ bool last;
-  const double s = block_total(acc, na, red);
   if (threadIdx.x < na) put_slot(&partials[(uint64_t)blockIdx.x * na + threadIdx.x], s);
   __syncthreads();
   if (threadIdx.x == 0)

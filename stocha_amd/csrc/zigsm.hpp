@@ -83,7 +83,7 @@ struct ZigFeed {
   __device__ __forceinline__ bool feed_eval(uint64_t u, const Tables* __restrict__ z, Real& out,
                                             bool tie) {
     if (tie) {  // exact fast-path test (normal.rs:26-29); out holds x
-      if ((u >> 11) < z->t[idx & 0xffu]) {
+      if ((u >> 11) < kZigT[idx & 0xffu]) {
         state = 0;
         return true;
       }
@@ -99,8 +99,8 @@ struct ZigFeed {
         const float e = __builtin_amdgcn_exp2f(-0.72134752044448170f * xw * xw);
         const float diff = y - e;
         if (fabsf(diff) > 4e-6f) return diff < 0.f;
-        const double x64 = ((double)(ua >> 11) * 0x1p-53) * z->x[l];
-        const double y64 = z->y[l] + ((double)(u >> 11) * 0x1p-53) * (z->y[l + 1] - z->y[l]);
+        const double x64 = ((double)(ua >> 11) * 0x1p-53) * kZigX[l];
+        const double y64 = kZigY[l] + ((double)(u >> 11) * 0x1p-53) * (kZigY[l + 1] - kZigY[l]);
         return y64 < exp(-0.5 * x64 * x64);
       } else {
         const double y0 = z->y[l], y1 = z->y[l + 1];
@@ -121,6 +121,124 @@ struct ZigFeed {
     state = 0;
     out = (idx & 256u) ? -t : t;
     return true;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// The same sampler for a whole wavefront at once (the queue schedule): the
+// per-lane state is held as lane masks -- wave-uniform 64-bit words, one bit
+// per lane -- so every state transition is scalar-unit work and the VALU
+// only computes the draw's values and compares (whose ballots are free: a
+// compare's result already is a lane mask).  The rarely needed work (a tied
+// fast test, a wedge uniform, the second draw of a tail pair) runs under
+// uniform branches, each only when some lane needs it, with its outcome
+// ballotted back into the masks.  Wave-collective: every lane of the
+// wavefront calls feed together; `act` names the lanes that consume a draw.
+// A lane's draws, values and accept/reject decisions are exactly ZigFeed's.
+__device__ __forceinline__ bool lane_in(uint64_t mask) {
+  return __builtin_amdgcn_inverse_ballot_w64(mask);
+}
+__device__ __forceinline__ uint64_t lanes(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+
+template <bool kFp32>
+struct ZigWave {
+  using Real = typename std::conditional<kFp32, float, double>::type;
+  using Tables = typename std::conditional<kFp32, ZigLds32, ZigLds>::type;
+  uint64_t wedge = 0;  // lanes whose next draw is the wedge uniform of attempt ua
+  uint64_t tail1 = 0;  // lanes whose next draw is the first of a tail pair
+  uint64_t tail2 = 0;  // lanes whose next draw is the second of the pair starting at ua
+  uint64_t neg = 0;    // sign of the layer-0 attempt that led each tail
+  uint64_t ua = 0;     // per lane: the missed attempt (wedge) or the pair's first draw
+
+  __device__ __forceinline__ uint64_t pending_mask() const { return wedge | tail2; }
+
+  // Consumes draw u on the lanes of `act`; returns the lanes whose draw
+  // completed a normal (in out).  `out` is written on every lane.
+  __device__ __forceinline__ uint64_t feed(uint64_t u, const Tables* __restrict__ z, Real& out,
+                                           uint64_t act) {
+    const uint32_t i = (uint32_t)u & 0x1ffu;
+    const uint32_t hi = (uint32_t)(u >> 32);
+    uint32_t th;
+    uint64_t sg;
+    if constexpr (kFp32) {
+      const ZigFast32 f = z->fast32[i];
+      th = f.th;
+      out = hi32_to_f32(u) * f.xs;
+      sg = lanes(f.xs < 0.f);  // the table folds the sign bit (bit 8) into X
+    } else {
+      const ZigFast f = z->fast[i];
+      th = f.th;
+      out = ((double)(u >> 11) * 0x1p-53) * f.x;
+      sg = lanes(f.x < 0.0);
+    }
+    const uint64_t lt = lanes(hi < th), eq = lanes(hi == th), l0 = lanes((i & 0xffu) == 0);
+    const uint64_t at = act & ~(wedge | tail1 | tail2);
+    const uint64_t take = at & lt;
+    // a tie on the high word (one draw in 2^32) is booked as a miss and
+    // settled with the full threshold below
+    const uint64_t tie = at & eq, miss = at & ~lt;
+    const uint64_t wl = act & wedge, t2 = act & tail2, t1 = act & tail1;
+    ua = lane_in(miss | t1) ? u : ua;
+    neg = (neg & ~miss) | (miss & sg);
+    wedge = (wedge & ~act) | (miss & ~l0);
+    tail1 = (tail1 & ~act) | (miss & l0);
+    tail2 = (tail2 & ~act) | t1;
+    uint64_t got = take;
+    if ((tie | wl | t2) != 0) got |= slow(u, z, out, tie, wl, t2);
+    return got;
+  }
+
+  __device__ __forceinline__ uint64_t slow(uint64_t u, const Tables* __restrict__ z, Real& out,
+                                           uint64_t tie, uint64_t wl, uint64_t t2) {
+    uint64_t got = 0;
+    if (tie != 0) {  // exact fast-path test (normal.rs:26-29); out holds x
+      const uint64_t ok = tie & lanes((u >> 11) < kZigT[u & 0xffu]);
+      wedge &= ~ok;
+      tail1 &= ~ok;
+      got |= ok;
+    }
+    if (wl != 0) {  // wedge (normal.rs:36-42): u is the uniform, ua the attempt
+      // (every lane evaluates; indices are masked into the tables)
+      const uint32_t ia = (uint32_t)ua & 0x1ffu, l = ia & 0xffu;
+      Real xw;
+      uint64_t acc;
+      if constexpr (kFp32) {
+        xw = hi32_to_f32(ua) * z->fast32[ia].xs;  // the attempt's value
+        const float2 yd = z->ydy32[l];
+        const float y = fmaf(top24_to_f32(u) * 0x1p-24f, yd.y, yd.x);
+        const float e = __builtin_amdgcn_exp2f(-0.72134752044448170f * xw * xw);
+        const float diff = y - e;
+        acc = lanes(diff < 0.f);
+        const uint64_t close = wl & ~lanes(fabsf(diff) > 4e-6f);
+        if (close != 0) {  // inside the guard band: the FP64 test decides
+          const double x64 = ((double)(ua >> 11) * 0x1p-53) * kZigX[l];
+          const double y64 =
+              kZigY[l] + ((double)(u >> 11) * 0x1p-53) * (kZigY[l + 1] - kZigY[l]);
+          acc = (acc & ~close) | (close & lanes(y64 < exp(-0.5 * x64 * x64)));
+        }
+      } else {
+        xw = ((double)(ua >> 11) * 0x1p-53) * z->fast[ia].x;
+        const double y0 = z->y[l], y1 = z->y[l + 1];
+        const double y = y0 + ((double)(u >> 11) * 0x1p-53) * (y1 - y0);
+        acc = lanes(y < exp(-0.5 * xw * xw));
+      }
+      out = lane_in(wl) ? xw : out;  // (feed has already left the wedge state)
+      got |= wl & acc;
+    }
+    if (t2 != 0) {  // tail (normal.rs:54-67): u is the pair's second draw
+      Real t;
+      bool ok;
+      if constexpr (kFp32) {
+        ok = zig_tail_pair32(ua, u, &t);
+      } else {
+        ok = zig_tail_pair64(ua, u, &t);
+      }
+      const uint64_t okm = t2 & lanes(ok);
+      tail1 |= t2 & ~okm;  // rejected: the next draw starts a new pair
+      out = lane_in(okm) ? (lane_in(neg) ? -t : t) : out;
+      got |= okm;
+    }
+    return got;
   }
 };
 
