@@ -17,6 +17,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 torch = None
 if os.environ.get("STOCHA_AMD_TORCH", "1") != "0":
     try:  # share torch's HIP runtime when it is present (see module docstring)
@@ -106,6 +108,39 @@ def load():
         fn.restype = res
     _lib = lib
     return lib
+
+
+# Results below this size are copied through a pageable buffer: cheaper than
+# the pinned allocator's bookkeeping.
+_PINNED_MIN = 1 << 20
+_pinned = None
+
+
+def _pinned_available() -> bool:
+    global _pinned
+    if _pinned is None:
+        try:
+            _pinned = torch is not None and bool(torch.cuda.is_available())
+        except Exception:  # pragma: no cover
+            _pinned = False
+    return _pinned
+
+
+def host_array(shape, dtype=np.float64):
+    """A C-contiguous float64 host array to receive a result from the device.
+
+    Large results go to page-locked memory from torch's caching host allocator
+    when torch's HIP runtime is the one in use: the copy back then runs at
+    PCIe DMA speed instead of through a pageable staging buffer, and the
+    allocator reuses freed blocks instead of pinning pages again.  The array
+    keeps its block alive.  Plain NumPy memory otherwise."""
+    shape = tuple(int(v) for v in shape)
+    n = 1
+    for v in shape:
+        n *= v
+    if (np.dtype(dtype) == np.float64 and n * 8 >= _PINNED_MIN and _pinned_available()):
+        return torch.empty(shape, dtype=torch.float64, pin_memory=True).numpy()
+    return np.empty(shape, dtype=dtype)
 
 
 def exported_names():

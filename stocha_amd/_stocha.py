@@ -72,7 +72,7 @@ def gbm(s0, mu, sigma, t, steps, n_paths, seed=42, antithetic=False, q=0.0):
     # out=NULL validates only, so a bad argument never allocates the output
     L.check(lib.stocha_gbm(s0, mu, sigma, t, steps, n_paths, seed, int(antithetic), q, None,
                            L.STOCHA_HOST, eb.buf, eb.n), eb)
-    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    out = L.host_array((n_paths, steps + 1))
     rc = lib.stocha_gbm(s0, mu, sigma, t, steps, n_paths, seed, int(antithetic), q,
                         out.ctypes.data, L.STOCHA_HOST, eb.buf, eb.n)
     L.check(rc, eb)
@@ -91,7 +91,7 @@ def heston(s0, v0, mu, kappa, theta, xi, rho, t, steps, n_paths, seed=42, scheme
     sch = scheme.encode()
     L.check(lib.stocha_heston(*vals, steps, n_paths, seed, sch, None, L.STOCHA_HOST, eb.buf,
                               eb.n), eb)
-    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    out = L.host_array((n_paths, steps + 1))
     rc = lib.stocha_heston(*vals, steps, n_paths, seed, sch, out.ctypes.data, L.STOCHA_HOST,
                            eb.buf, eb.n)
     L.check(rc, eb)
@@ -107,7 +107,7 @@ def merton_jump_diffusion(s0, mu, sigma, lambda_, mu_j, sigma_j, t, steps, n_pat
     lib, eb = L.load(), L.ErrBuf()
     L.check(lib.stocha_merton_jump_diffusion(*vals, steps, n_paths, seed, None, L.STOCHA_HOST,
                                              eb.buf, eb.n), eb)
-    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    out = L.host_array((n_paths, steps + 1))
     rc = lib.stocha_merton_jump_diffusion(*vals, steps, n_paths, seed, out.ctypes.data,
                                           L.STOCHA_HOST, eb.buf, eb.n)
     L.check(rc, eb)
@@ -135,7 +135,7 @@ def multi_gbm(s0, mu, sigma, corr, t, steps, n_paths, seed=42, antithetic=False)
                                     eb.n)
 
     L.check(call(None), eb)
-    out = np.empty((n_paths, steps + 1, d), dtype=np.float64)
+    out = L.host_array((n_paths, steps + 1, d))
     L.check(call(out.ctypes.data), eb)
     return out
 
@@ -281,7 +281,7 @@ def _greeks_from_prices(greeks, prices, bumps):
 
 
 def _terminal_prices(mid, vals, scheme, n_steps, n_paths, seed):
-    out = np.empty(n_paths, dtype=np.float64)
+    out = L.host_array((n_paths,))
     lib, eb = L.load(), L.ErrBuf()
     rc = lib.stocha_terminal_prices(mid, _pd(vals), len(vals),
                                     None if scheme is None else scheme.encode(), n_steps,
@@ -455,7 +455,7 @@ class RNG:
     def uniform(self, size):
         """lib.rs:108-112: size draws of next_f64."""
         size = _uint(size, "size")
-        out = np.empty(size, dtype=np.float64)
+        out = L.host_array((size,))
         st, inc, new = self._args()
         lib, eb = L.load(), L.ErrBuf()
         L.check(lib.stocha_rng_uniform(st, inc, size, out.ctypes.data, L.STOCHA_HOST, new,
@@ -464,7 +464,7 @@ class RNG:
         return out
 
     def _normals(self, size, loc, scale, scaled):
-        out = np.empty(size, dtype=np.float64)
+        out = L.host_array((size,))
         st, inc, new = self._args()
         lib, eb = L.load(), L.ErrBuf()
         L.check(lib.stocha_rng_normal(st, inc, size, loc, scale, int(scaled), out.ctypes.data,
@@ -534,7 +534,7 @@ def hull_white(r0, a, theta, sigma, t, steps, n_paths, seed=42):
     lib, eb = L.load(), L.ErrBuf()
     L.check(lib.stocha_hull_white(*vals, steps, n_paths, seed, None, L.STOCHA_HOST, eb.buf, eb.n),
             eb)
-    out = np.empty((n_paths, steps + 1), dtype=np.float64)
+    out = L.host_array((n_paths, steps + 1))
     L.check(lib.stocha_hull_white(*vals, steps, n_paths, seed, out.ctypes.data, L.STOCHA_HOST,
                                   eb.buf, eb.n), eb)
     return out
@@ -552,7 +552,7 @@ def student_t_copula(corr, nu, n_samples, seed=42):
     lib, eb = L.load(), L.ErrBuf()
     L.check(lib.stocha_student_t_copula(_pd(c), c.shape[0], c.shape[1], nu, n_samples, seed,
                                         None, L.STOCHA_HOST, eb.buf, eb.n), eb)
-    out = np.empty((n_samples, c.shape[0]), dtype=np.float64)
+    out = L.host_array((n_samples, c.shape[0]))
     L.check(lib.stocha_student_t_copula(_pd(c), c.shape[0], c.shape[1], nu, n_samples, seed,
                                         out.ctypes.data, L.STOCHA_HOST, eb.buf, eb.n), eb)
     return out
@@ -569,7 +569,7 @@ def gaussian_copula(corr, n_samples, seed=42):
     lib, eb = L.load(), L.ErrBuf()
     L.check(lib.stocha_gaussian_copula(_pd(c), c.shape[0], c.shape[1], n_samples, seed, None,
                                        L.STOCHA_HOST, eb.buf, eb.n), eb)
-    out = np.empty((n_samples, c.shape[0]), dtype=np.float64)
+    out = L.host_array((n_samples, c.shape[0]))
     L.check(lib.stocha_gaussian_copula(_pd(c), c.shape[0], c.shape[1], n_samples, seed,
                                        out.ctypes.data, L.STOCHA_HOST, eb.buf, eb.n), eb)
     return out
