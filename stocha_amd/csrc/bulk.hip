@@ -30,13 +30,13 @@
 //               records where it ends (the draws consumed).
 // Values are the reference's (ZigFeed consumes exactly the reference's draws);
 // the host advances the generator by the draws consumed.
-#include <hipcub/hipcub.hpp>
 
 #include <math.h>
 
 #include <vector>
 
 #include "host.hpp"
+#include "scan.hpp"
 #include "multi.hpp"
 #include "zigsm.hpp"
 
@@ -253,9 +253,7 @@ int decode_normals(const uint64_t state[2], const uint64_t inc[2], uint64_t n, d
     if (n_chunks > 0x7fffffffull) return err.set(STOCHA_EINVAL, "size too large");
     const unsigned grid = (unsigned)((n_chunks + kBulkBlock - 1) / kBulkBlock);
     const size_t nc = n_chunks;
-    size_t tmp = 0;
-    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const uint64_t*)nullptr,
-                                                (uint64_t*)nullptr, (int)nc, st));
+    const size_t tmp = 8 * scan_tmp_words(nc);
     const size_t extra = al256(4 * nc) + 6 * al256(8 * nc) + 256 + al256(tmp);
     Workspace* ws = nullptr;
     if (int rc = workspace(al256(front) + extra, 0, &ws, err)) return rc;
@@ -302,7 +300,8 @@ int decode_normals(const uint64_t state[2], const uint64_t inc[2], uint64_t n, d
       cur ^= 1;
     }
     // prev: the exits whose entries the final resolve used
-    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, off, (int)nc, st));
+    if (int rc = exclusive_sum_u64(cnt, off, nc, static_cast<uint64_t*>(scan_tmp), st, err))
+      return rc;
     uint64_t last_off = 0, last_cnt = 0;
     STOCHA_HIP(hipMemcpyAsync(&last_off, off + nc - 1, 8, hipMemcpyDeviceToHost, st));
     STOCHA_HIP(hipMemcpyAsync(&last_cnt, cnt + nc - 1, 8, hipMemcpyDeviceToHost, st));
@@ -964,9 +963,7 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     if (n_tot > (1ull << 40)) return err.set(STOCHA_EINVAL, "size too large");
     const size_t nc = n_chunks;
     const uint64_t n_blk = (nc + kBulkBlock - 1) / kBulkBlock;
-    size_t tmp = 0;
-    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const uint64_t*)nullptr,
-                                                (uint64_t*)nullptr, (int)nc, st));
+    const size_t tmp = 8 * scan_tmp_words(nc);
     const size_t sz[] = {8 * n_tot,     2 * n_pos,     sizeof(TMap) * nc, 0,
                          kTChunk * nc,  nc,            8 * nc,            8 * nc,
                          256,           tmp,           8 * n_samples,     8 * n,
@@ -1012,7 +1009,8 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     t_entry_kernel<<<(unsigned)n_blk, kBulkBlock, 0, st>>>(maps, xin, cnt, n_chunks, entry,
                                                             cnt_true, flag);
     STOCHA_HIP(hipGetLastError());
-    STOCHA_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt_true, off, (int)nc, st));
+    if (int rc = exclusive_sum_u64(cnt_true, off, nc, static_cast<uint64_t*>(scan_tmp), st, err))
+      return rc;
     unsigned h_flag = 0;
     uint64_t last_off = 0, last_cnt = 0;
     STOCHA_HIP(hipMemcpyAsync(&h_flag, flag, 4, hipMemcpyDeviceToHost, st));

@@ -116,29 +116,34 @@ struct Heston64 {
 // Heston QE, log-price part (heston.rs:152-157); the variance recursion runs
 // in the draw source (HestonQeSrc), which hands over V(t+dt) with the
 // step's asset normal.  v tracks the same recursion (v0, then each V(t+dt)).
-// The state is ln S: the reference takes S = exp(ln S) every step and ln S
-// back from it on the next, and the pair is dropped here (S is formed only
-// when read).  log(exp(x)) returns x itself wherever |x| is above about 1,
-// since the rounding of exp then moves the logarithm by less than half an
-// ulp of x; nearer 0 it can differ by that 1e-16 relative, so a path differs
-// from the reference's by at most steps x 1e-16 relative.
-struct HestonQe64 {
+// kRoundTrip (the path matrices): the state is S and every step takes
+// ln S and S = exp(clamp(log_s)) as the reference does, so each stored value
+// is the reference's to the device's log/exp.  Without it (terminal prices,
+// fused payoffs, greeks_fd) the state is ln S and the pair is dropped, S
+// being formed only when read: log(exp(x)) differs from x by at most half an
+// ulp of ln S per step, so S_T moves by at most steps x 1e-16 relative.
+template <bool kRoundTrip>
+struct HestonQeModel {
   static constexpr bool kFp32 = false;
   HestonQeConst c;
-  double ls, v;
-  __device__ __forceinline__ HestonQe64(const HestonQeConst& k, bool)
-      : c(k), ls(log(k.s0)), v(k.v0) {}
+  double x, v;  // kRoundTrip: S; otherwise ln S
+  __device__ __forceinline__ HestonQeModel(const HestonQeConst& k, bool)
+      : c(k), x(kRoundTrip ? k.s0 : log(k.s0)), v(k.v0) {}
   __device__ __forceinline__ void apply_qe(double v_next, double z1) {
     const double v_plus = fmax(v, 0.0);
     double var_term = c.k3s * v_plus + c.k4s * v_next;
     var_term = fmax(var_term, 0.0);
+    const double ls = kRoundTrip ? log(x) : x;
     double log_s = ls + c.k0s + c.k1s * v_plus + c.k2s * v_next + sqrt(var_term) * z1;
     // f64::clamp(-500, 500) keeps a NaN
-    ls = log_s < -500.0 ? -500.0 : (log_s > 500.0 ? 500.0 : log_s);
+    log_s = log_s < -500.0 ? -500.0 : (log_s > 500.0 ? 500.0 : log_s);
+    x = kRoundTrip ? exp(log_s) : log_s;
     v = v_next;
   }
-  __device__ __forceinline__ double terminal() const { return exp(ls); }
+  __device__ __forceinline__ double terminal() const { return kRoundTrip ? x : exp(x); }
 };
+using HestonQe64 = HestonQeModel<false>;
+using HestonQe64Path = HestonQeModel<true>;
 
 // hull_white.rs:47-49: r(t+dt) = r(t) e + mean_reversion + vol z
 struct HullWhite64 {

@@ -104,7 +104,7 @@ int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n
           *(const HullWhiteConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     case kHestonQe:
-      paths_kernel<HestonQeSrc<HestonQe64>, HestonQe64, HestonQeConst, 4>
+      paths_kernel<HestonQeSrc<HestonQe64Path>, HestonQe64Path, HestonQeConst, 4>
           <<<grid, block, 0, st>>>(*(const HestonQeConst*)mconst, sd, n_rows, base_rows, st32,
                                    d_out);
       break;

@@ -72,3 +72,23 @@ def test_qe_terminal_prices_match_oracle(gpu, orc):
     dev = S._terminal_prices(1, vals, "qe", 40, 900, 42)
     ref = orc.terminals(3, list(vals[:7]), 1.0, 40, 900, seed=42)
     np.testing.assert_allclose(dev, ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("s0", [1.0, math.exp(0.5), 0.37])
+def test_qe_paths_near_unit_spot(gpu, orc, s0):
+    # ln S near 0: the path matrix takes the reference's ln / exp pair every
+    # step, so each value is the reference's up to the device log / exp
+    kw = dict(BASE, s0=s0)
+    dev = gpu.heston(**kw, steps=300, n_paths=1200, seed=5, scheme="qe")
+    ref = orc.heston_qe(**kw, steps=300, n_paths=1200, seed=5)
+    np.testing.assert_allclose(dev, ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("s0", [1.0, math.exp(0.5)])
+def test_qe_terminals_near_unit_spot(gpu, orc, s0):
+    # the terminal forms keep ln S as their state: within steps x 1e-16
+    from stocha_amd import _stocha as S
+    vals = np.array([s0, 0.04, 0.05, 2.0, 0.04, 0.3, -0.7, 1.0])
+    dev = S._terminal_prices(1, vals, "qe", 300, 1500, 5)
+    ref = orc.terminals(3, list(vals[:7]), 1.0, 300, 1500, seed=5)
+    np.testing.assert_allclose(dev, ref, rtol=1e-12, atol=0)

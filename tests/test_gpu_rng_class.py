@@ -80,3 +80,13 @@ def test_reference_rng_checks(gpu):
     assert abs(z.mean()) < 0.005 and abs(z.std() - 1.0) < 0.01
     _, p = stats.kstest(gpu.RNG(seed=123).standard_normal(size=100_000), "norm")
     assert p > 0.01
+
+
+def test_standard_normal_many_scan_tiles(gpu, orc):
+    # ~1.1M decoding chunks: the device prefix sum of the chunk counts has
+    # more than one tile (1024) of tile sums, so its carry across tiles is used
+    n = 70_000_000
+    print(f"rng normals n={n}", flush=True)
+    r, o = gpu.RNG(31), orc.Rng(31)
+    _check_normals(r.standard_normal(n), o.normal(n))
+    assert r._state == o.state()[0]
