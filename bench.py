@@ -308,6 +308,26 @@ def extras(S, out):
                                           "bytes": gbytes}}
     log("gbm paths done")
 
+    # multi_gbm path tensor, d = 8 correlated assets, 100,000 x 252, FP64
+    # (multi_gbm.rs): 100,000 x 253 x 8 values written
+    mrng = np.random.default_rng(SEED)
+    ma = mrng.standard_normal((8, 8))
+    mcv = ma @ ma.T / 8 + 0.1 * np.eye(8)
+    mdg = np.sqrt(np.diag(mcv))
+    mkw = dict(s0=list(mrng.uniform(50, 150, 8)), mu=list(mrng.uniform(0.0, 0.08, 8)),
+               sigma=list(mrng.uniform(0.1, 0.4, 8)), corr=mcv / np.outer(mdg, mdg), t=1.0,
+               steps=252, n_paths=100_000, seed=SEED)
+    tm, km = time_calls(lambda: S.multi_gbm(**mkw), 3, 1, S)
+    mbytes = 100_000 * 253 * 8 * 8
+    out["multi_gbm_d8_100k"] = {"kernel_ms": km, "wall_ms_incl_d2h": tm * 1e3,
+                                "asset_steps_per_s_kernel": 100_000 * 252 * 8 / (km * 1e-3),
+                                "roofline": {"bound": "hbm",
+                                             "achieved": mbytes / (km * 1e-3) / 1e9,
+                                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                             "frac": mbytes / (km * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                             "bytes": mbytes}}
+    log("multi_gbm done")
+
     # bump-and-revalue Greeks, GBM call 1,048,576 x 252, all five Greeks:
     # 8 bump scenarios in FP64 on shared draws (one launch), against the
     # same 8 scenarios priced one launch each (FP64 fused European kernel)

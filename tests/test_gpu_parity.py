@@ -108,6 +108,38 @@ def test_multi_gbm_paths(gpu, orc):
         np.testing.assert_allclose(dev, ref, rtol=RTOL64, atol=0)
 
 
+def _corr(d, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((d, d))
+    c = a @ a.T / d + 0.1 * np.eye(d)
+    dg = np.sqrt(np.diag(c))
+    return c / np.outer(dg, dg), rng
+
+
+@pytest.mark.parametrize("d,n,steps,anti", [(5, 301, 37, False), (8, 1000, 20, True),
+                                            (12, 257, 9, False), (16, 513, 11, True),
+                                            (17, 300, 6, False), (24, 129, 5, True)])
+def test_multi_gbm_tile_shapes(gpu, orc, d, n, steps, anti):
+    # d <= 16 stages whole steps in the LDS tile (d = 12, 16: one step per
+    # flush; 5, 8: several); d > 16 writes rows directly; ragged last blocks
+    corr, rng = _corr(d, d)
+    kw = dict(s0=list(rng.uniform(50, 150, d)), mu=list(rng.uniform(0.0, 0.08, d)),
+              sigma=list(rng.uniform(0.1, 0.4, d)), corr=corr, t=1.0, steps=steps, n_paths=n,
+              seed=d + 1, antithetic=anti)
+    dev = gpu.multi_gbm(**kw)
+    ref = orc.multi_gbm(**kw)
+    assert dev.shape == (n, steps + 1, d)
+    np.testing.assert_allclose(dev, ref, rtol=RTOL64, atol=0)
+
+
+def test_multi_gbm_d8_100k(gpu, orc):
+    corr, rng = _corr(8, 42)
+    kw = dict(s0=list(rng.uniform(50, 150, 8)), mu=list(rng.uniform(0.0, 0.08, 8)),
+              sigma=list(rng.uniform(0.1, 0.4, 8)), corr=corr, t=1.0, steps=252,
+              n_paths=100_000, seed=42)
+    np.testing.assert_allclose(gpu.multi_gbm(**kw), orc.multi_gbm(**kw), rtol=RTOL64, atol=0)
+
+
 def test_multi_gbm_d1_equals_gbm(gpu):
     kw = dict(t=1.0, steps=60, n_paths=500, seed=11)
     m = gpu.multi_gbm(s0=[100.0], mu=[0.05], sigma=[0.2], corr=np.eye(1), **kw)
