@@ -8,13 +8,13 @@
 // Cholesky factor L
 //     log S_T,a = log s0_a + steps*drift_a + diffusion_a * (L W)_a,
 //     W = sum over steps of z_t,
-// so a path keeps the running sum W (one LDS column per lane, d entries) and
-// forms E = L W once at the end (VALU dot products with L read through the
-// scalar cache; or, in the FP32 form, v_mfma_f32_16x16x4f32 tiles over a
-// wave's 64 paths).  This equals the reference's per-step L z_t products in
-// exact arithmetic; rounding differs by the reassociation only.  The FP32
-// form runs in two kernels (the draws, then payoff and adjoint from W kept
-// asset-major in the workspace); the FP64 form in one.
+// so a path keeps the running sum W (d values in registers) and forms E = L W
+// once at the end (VALU dot products with L read through the scalar cache;
+// or, in the FP32 form, v_mfma_f32_16x16x4f32 tiles over a wave's 64
+// paths).  This equals the reference's per-step L z_t products in exact
+// arithmetic; rounding differs by the reassociation only.  Both precisions
+// run in two kernels: the draws, then payoff and adjoint from W kept
+// asset-major in the workspace.
 //
 // Adjoint of one path (reverse mode through the payoff):
 //   B = sum_a w_a S_a,  Bbar = 1{B > K},  Sbar_a = w_a Bbar
@@ -25,8 +25,9 @@
 // pass of the autograd wrapper replays the streams with this kernel instead
 // of storing paths.
 //
-// Each lane runs the flat draw-feed state machine (zigsm.hpp) over its
-// path's steps*d normals; lanes re-synchronise only at path boundaries.
+// The draws run on the queue schedule (drivers.hpp): each lane's steps*d
+// normals pass through a per-lane ring, and the wavefront takes them out in
+// chunks of the same assets on every lane.
 #include "host.hpp"
 #include "drivers.hpp"
 
@@ -171,97 +172,186 @@ __device__ __forceinline__ void basket_mfma_epilogue_w(const BasketDev& bd, cons
   basket_mfma_payoff(bd, acc, wave_path0, psum, psq, gd, gm, gv, greeks);
 }
 
-// accumulator layout per block: [0] sum payoff, [1] sum payoff^2,
-// then d x {dP/ds0, dP/dmu, dP/dsigma}
-template <bool kFp32, int kBlock, bool kGreeks, bool kMfma>
-__global__ __launch_bounds__(kBlock) void basket_kernel(const BasketDev bd, const StreamDesc sd,
-                                                        double* __restrict__ partials) {
-  static_assert(!kMfma || kFp32, "the matrix-core epilogue is FP32");
-  using Real = typename std::conditional<kFp32, float, double>::type;
-  __shared__ ZigLds zig;
-  __shared__ Real W[kMaxAssets][kBlock];
-  __shared__ double red[kBlock / 64][2 + 3 * kMaxAssets];
+// ------------------------------------------------------------- two phases --
+// Phase 1: the draws, on the queue schedule (drivers.hpp): one path per lane,
+// the lanes' normals pass through per-lane rings in LDS and the wave takes
+// them out in chunks -- chunk c of a step is assets kChunk*c .. kChunk*c+n-1,
+// n = min(kChunk, d - kChunk*c) -- so that the asset of every normal taken is
+// the same on all lanes of the wave.  W (the running sum of each asset's
+// normals) then lives in registers, indexed by constants under a
+// wave-uniform branch on the chunk; LDS holds only the tables and the rings
+// (kSumsSlots entries per path), so occupancy is set by the registers (W is
+// kChunk*NC values).  W goes to ws[a * np + p] (asset-major, so both these
+// stores and the loads of phase 2 are coalesced).  Phase 2, the payoff and
+// adjoint, is a separate kernel so that none of its LDS or registers count
+// against the draws.  Both run over slices of at most kSlicePaths paths, which
+// bounds the workspace at kSlicePaths*d values.
+constexpr int kSumsBlock = 256;
+constexpr int kChunk = 8;
+// Ring entries per lane.  FP32: 32 lets lanes that draw ahead bank normals
+// across chunks (16 costs 6% more draw-loop passes and 45% more slow-block
+// passes); FP64: 16, which keeps a 256-thread block's LDS at 51 KB.
+template <bool kFp32, int NC>
+constexpr uint32_t kSumsSlots = kFp32 ? 32 : 16;
+constexpr int kRegBlock = 256;
+constexpr uint64_t kSlicePaths = 1ull << 22;
+constexpr uint64_t kPayoffBlocks = 4096;  // phase-2 grid cap (grid-stride beyond)
+
+// waves per SIMD the phase-1 register budget allows (W is kChunk*NC values,
+// two registers each in FP64, beside the draw loop's own)
+template <bool kFp32, int NC>
+constexpr int sums_waves() {
+  return kFp32 ? (NC > 4 ? 4 : (NC > 2 ? 5 : 6)) : (NC > 4 ? 2 : 3);
+}
+
+// Paths p0 .. p0+np-1 of the launch (np = bd.n_paths of the slice); lane
+// p - p0 of the grid runs path p.
+template <bool kFp32, int NC>
+__global__ __launch_bounds__(kSumsBlock) __attribute__((amdgpu_waves_per_eu(
+    sums_waves<kFp32, NC>()))) void basket_sums_kernel(const BasketDev bd, const StreamDesc sd,
+                                                       const uint64_t p0,
+                                                       typename ZigWave<kFp32>::Real* __restrict__
+                                                           ws) {
+  using Src = GbmSrc<kFp32>;
+  using Real = typename Src::Real;
+  __shared__ typename Src::Tables zig;
+  constexpr uint32_t kSlots = kSumsSlots<kFp32, NC>;
+  __shared__ Real q[kSlots][kSumsBlock];
   zig_load(&zig);
   __syncthreads();
+  const uint64_t np = bd.n_paths;
+  // whole wavefronts past the last path leave (wave-uniform; no barrier below)
+  if ((uint64_t)blockIdx.x * kSumsBlock + (threadIdx.x & ~63u) >= np) return;
+  const uint64_t p = (uint64_t)blockIdx.x * kSumsBlock + threadIdx.x;
+  const bool valid = p < np;
+  Src src(bd);
+  Pcg g{stream_at(&sd, p0 + (valid ? p : 0))};  // lanes past the end replay a path, unstored
+  const U128 incv = vgpr_increment(sd.inc);
+  const uint32_t d = bd.d;
+  const uint32_t nc = (d + kChunk - 1) / kChunk;  // <= NC
+  Real W[kChunk * NC];
+#pragma unroll
+  for (int i = 0; i < kChunk * NC; ++i) W[i] = (Real)0;
+  uint32_t head = 0, cnt = 0;
+  for (uint32_t step = 0; step < bd.steps; ++step) {
+    for (uint32_t c = 0; c < nc; ++c) {
+      const uint32_t n = min((uint32_t)kChunk, d - kChunk * c);
+      queue_fill<Src, kSlots, kSumsBlock>(src, g, incv, &zig, q, head, cnt, n);
+      // all kChunk ring reads are issued before the first add (entries past
+      // n are read and dropped); a whole chunk from a chunk-aligned head
+      // reads kChunk consecutive rows at constant offsets
+      Real z[kChunk];
+      const bool whole = n == (uint32_t)kChunk && (head & (kChunk - 1)) == 0;
+      if (whole) {
+        const Real* row = &q[head][threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) z[j] = row[j * kSumsBlock];
+      } else {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const Real v = q[(head + j) & (kSlots - 1)][threadIdx.x];
+          z[j] = (uint32_t)j < n ? v : (Real)0;
+        }
+      }
+#pragma unroll
+      for (int cc = 0; cc < NC; ++cc) {
+        if (cc == (int)c) {
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) W[kChunk * cc + j] += z[j];
+        }
+      }
+      head = (head + n) & (kSlots - 1);
+      cnt -= n;
+    }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int a = 0; a < kChunk * NC; ++a)
+    if ((uint32_t)a < d) ws[(uint64_t)a * np + p] = W[a];
+}
 
+// Phase 2: payoff and adjoint from W (ws, asset-major over the slice's np =
+// bd.n_paths paths).  Each wave takes 64 consecutive paths, grid-stride.
+// kMfma (FP32): E = W^T L^T on v_mfma_f32_16x16x4f32 (basket_mfma_epilogue_w,
+// A operands loaded straight from ws); otherwise per lane
+// E_i = sum_{k<=i} L[i][k] W[k] on the VALU with L through the scalar cache
+// (FP32: fmaf over the FP32 factor; FP64: products then sums, in k order).
+// accumulator layout per block: [0] sum payoff, [1] sum payoff^2,
+// then d x {dP/ds0, dP/dmu, dP/dsigma}
+template <class Real>
+struct GlobalW {
+  const Real* ws;
+  uint64_t n;
+  __device__ __forceinline__ Real operator()(int k, uint64_t path) const {
+    return path < n ? ws[(uint64_t)k * n + path] : (Real)0;
+  }
+};
+
+template <class Real>
+__device__ __forceinline__ Real basket_e(const BasketDev& bd, const GlobalW<Real>& W, uint32_t i,
+                                         uint64_t p) {
+  Real e = (Real)0;
+  for (uint32_t k = 0; k <= i; ++k) {
+    if constexpr (std::is_same<Real, float>::value)
+      e = fmaf(bd.Lf[i * 64 + k], W((int)k, p), e);
+    else
+      e += bd.L[i * bd.d + k] * W((int)k, p);
+  }
+  return e;
+}
+
+template <class Real, bool kGreeks, bool kMfma>
+__global__ __launch_bounds__(kRegBlock) void basket_payoff_kernel(const BasketDev bd,
+                                                                  const Real* __restrict__ ws,
+                                                                  double* __restrict__ partials) {
+  static_assert(!kMfma || std::is_same<Real, float>::value, "the matrix-core epilogue is FP32");
+  __shared__ double red[kRegBlock / 64][2 + 3 * kMaxAssets];
   const uint32_t d = bd.d;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const GlobalW<Real> W{ws, bd.n_paths};
   double psum = 0.0, psq = 0.0;
-  double gd = 0.0, gm = 0.0, gv = 0.0;  // VALU epilogue: lane a owns asset a's sums
-  double md[4] = {0, 0, 0, 0}, mm[4] = {0, 0, 0, 0}, mv[4] = {0, 0, 0, 0};  // MFMA epilogue
-  const U128 inc = sd.inc;
-  const uint64_t total = (uint64_t)bd.steps * d;
-
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < bd.n_paths;
-       base += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t p = base + threadIdx.x;
-    const bool valid = p < bd.n_paths;
-    for (uint32_t a = 0; a < d; ++a) W[a][threadIdx.x] = (Real)0;
-    Pcg g{stream_at(&sd, valid ? p : 0)};
-    ZigFeed<kFp32> zf;
-    uint32_t a = 0;
-    for (uint64_t k = 0; k < total;) {
-      const uint64_t u = g.next(inc);
-      Real x;
-      if (zf.feed(u, &zig, x)) {
-        W[a][threadIdx.x] += x;
-        a = (a + 1 == d) ? 0 : a + 1;
-        ++k;
-      }
-    }
+  double gd = 0.0, gm = 0.0, gv = 0.0;  // VALU: lane a owns asset a's sums
+  double md[4] = {0, 0, 0, 0}, mm[4] = {0, 0, 0, 0}, mv[4] = {0, 0, 0, 0};  // MFMA
+  for (uint64_t wave_path0 = (uint64_t)blockIdx.x * kRegBlock + (threadIdx.x & ~63u);
+       wave_path0 < bd.n_paths; wave_path0 += (uint64_t)gridDim.x * kRegBlock) {
     if constexpr (kMfma) {
-      const uint64_t wp0 = base + (threadIdx.x & ~63);
-      const auto lds_w = [&](int k, uint64_t path) { return W[k][(int)(path - base)]; };
-      basket_mfma_epilogue_w(bd, lds_w, wp0, psum, psq, md, mm, mv,
-                                   kGreeks);
-      continue;
-    }
-    // epilogue: E = L W, S_T, basket payoff (per lane)
-    double B = 0.0;
-    for (uint32_t i = 0; i < d; ++i) {
-      Real e = (Real)0;
-      for (uint32_t k = 0; k <= i; ++k) {
-        if constexpr (kFp32)
-          e = fmaf(bd.Lf[i * 64 + k], W[k][threadIdx.x], e);
-        else
-          e += bd.L[i * d + k] * W[k][threadIdx.x];
-      }
-      const double s = bd.s0[i] * exp(bd.lmean[i] + bd.diff[i] * (double)e);
-      B += bd.w[i] * s;
-    }
-    const double pay = valid && B > bd.strike ? B - bd.strike : 0.0;
-    psum += pay;
-    psq += pay * pay;
-    if constexpr (kGreeks) {
-      const bool itm = valid && B > bd.strike;
+      basket_mfma_epilogue_w(bd, W, wave_path0, psum, psq, md, mm, mv, kGreeks);
+    } else {
+      const uint64_t p = wave_path0 + lane;
+      const bool valid = p < bd.n_paths;
+      double B = 0.0;
+#pragma unroll 1
       for (uint32_t i = 0; i < d; ++i) {
-        double cd = 0.0, cm = 0.0, cv = 0.0;
-        if (itm) {
-          Real e = (Real)0;
-          for (uint32_t k = 0; k <= i; ++k) {
-            if constexpr (kFp32)
-              e = fmaf(bd.Lf[i * 64 + k], W[k][threadIdx.x], e);
-            else
-              e += bd.L[i * d + k] * W[k][threadIdx.x];
+        const Real e = basket_e(bd, W, i, p);
+        B += bd.w[i] * (bd.s0[i] * exp(bd.lmean[i] + bd.diff[i] * (double)e));
+      }
+      const bool itm = valid && B > bd.strike;
+      const double pay = itm ? B - bd.strike : 0.0;
+      psum += pay;
+      psq += pay * pay;
+      if constexpr (kGreeks) {
+#pragma unroll 1
+        for (uint32_t i = 0; i < d; ++i) {
+          double cd = 0.0, cm = 0.0, cv = 0.0;
+          if (itm) {
+            const Real e = basket_e(bd, W, i, p);
+            const double sb = bd.w[i] * (bd.s0[i] * exp(bd.lmean[i] + bd.diff[i] * (double)e));
+            cd = sb / bd.s0[i];
+            cm = sb * bd.t;
+            cv = sb * (-bd.sigma[i] * bd.t + bd.sqrt_dt * (double)e);
           }
-          const double s = bd.s0[i] * exp(bd.lmean[i] + bd.diff[i] * (double)e);
-          const double sb = bd.w[i] * s;  // Sbar_a * S_a
-          cd = sb / bd.s0[i];
-          cm = sb * bd.t;
-          cv = sb * (-bd.sigma[i] * bd.t + bd.sqrt_dt * (double)e);
-        }
-        cd = wave_sum_t(cd);
-        cm = wave_sum_t(cm);
-        cv = wave_sum_t(cv);
-        if ((uint32_t)lane == i) {
-          gd += cd;
-          gm += cm;
-          gv += cv;
+          cd = wave_sum_t(cd);
+          cm = wave_sum_t(cm);
+          cv = wave_sum_t(cv);
+          if ((uint32_t)lane == i) {
+            gd += cd;
+            gm += cm;
+            gv += cv;
+          }
         }
       }
     }
   }
-
   if constexpr (kMfma && kGreeks) {
     // lanes l, l^16, l^32, l^48 hold the same four assets: fold them, then
     // lane li (< 16) owns assets li, 16+li, 32+li, 48+li
@@ -273,155 +363,6 @@ __global__ __launch_bounds__(kBlock) void basket_kernel(const BasketDev bd, cons
       mm[nt] += __shfl_xor(mm[nt], 32, 64);
       mv[nt] += __shfl_xor(mv[nt], 16, 64);
       mv[nt] += __shfl_xor(mv[nt], 32, 64);
-    }
-  }
-  // block reduction: price sums over all lanes, greek sums over waves
-  psum = wave_sum_t(psum);
-  psq = wave_sum_t(psq);
-  if (lane == 0) {
-    red[wave][0] = psum;
-    red[wave][1] = psq;
-  }
-  if constexpr (kGreeks && !kMfma) {
-    if ((uint32_t)lane < d) {
-      red[wave][2 + 3 * lane] = gd;
-      red[wave][3 + 3 * lane] = gm;
-      red[wave][4 + 3 * lane] = gv;
-    }
-  }
-  if constexpr (kGreeks && kMfma) {
-    if (lane < 16) {
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        const uint32_t a = 16 * nt + lane;
-        if (a < d) {
-          red[wave][2 + 3 * a] = md[nt];
-          red[wave][3 + 3 * a] = mm[nt];
-          red[wave][4 + 3 * a] = mv[nt];
-        }
-      }
-    }
-  }
-  __syncthreads();
-  const int na = kGreeks ? 2 + 3 * (int)d : 2;
-  for (int i = threadIdx.x; i < na; i += kBlock) {
-    double s = 0.0;
-    for (int wv = 0; wv < kBlock / 64; ++wv) s += red[wv][i];
-    partials[(uint64_t)blockIdx.x * na + i] = s;
-  }
-}
-
-// ------------------------------------------------------- FP32, two phases --
-// Phase 1 (FP32): the draws.  One path per thread, the per-lane draw-feed
-// loop over its steps*d normals with W (the running sum of the normals of
-// each asset) in an LDS column; W then goes to ws[a * n + p] (asset-major,
-// so both these stores and the loads of phase 2 are coalesced).  W takes
-// 256 bytes of LDS per path, so a 512-thread block (146 KB with the tables)
-// is the most a CU holds: the payoff and adjoint run as a separate kernel
-// so that none of their LDS or registers count against this one.
-constexpr int kSumsBlock = 512;
-constexpr int kRegBlock = 256;
-
-__global__ __launch_bounds__(kSumsBlock) void basket_sums_kernel(const BasketDev bd,
-                                                                 const StreamDesc sd,
-                                                                 float* __restrict__ ws) {
-  __shared__ ZigLds zig;
-  __shared__ float W[kMaxAssets][kSumsBlock];
-  zig_load(&zig);
-  const uint32_t d = bd.d;
-  for (uint32_t a = 0; a < d; ++a) W[a][threadIdx.x] = 0.f;
-  __syncthreads();
-  const uint64_t p = (uint64_t)blockIdx.x * kSumsBlock + threadIdx.x;
-  if (p >= bd.n_paths) return;  // no block barrier below
-  Pcg g{stream_at(&sd, p)};
-  const U128 inc = sd.inc;
-  ZigFeed<true> zf;
-  const uint64_t total = (uint64_t)bd.steps * d;
-  uint32_t a = 0;
-  for (uint64_t k = 0; k < total;) {
-    const uint64_t u = g.next(inc);
-    float x;
-    if (zf.feed(u, &zig, x)) {
-      W[a][threadIdx.x] += x;
-      a = (a + 1 == d) ? 0 : a + 1;
-      ++k;
-    }
-  }
-  for (uint32_t k2 = 0; k2 < d; ++k2) ws[(uint64_t)k2 * bd.n_paths + p] = W[k2][threadIdx.x];
-}
-
-// Phase 2 (FP32): payoff and adjoint from W (ws, asset-major).  Each wave
-// takes 64 consecutive paths.  kMfma: E = W^T L^T on v_mfma_f32_16x16x4f32
-// (basket_mfma_epilogue, A operands loaded straight from ws); otherwise per
-// lane E_i = sum_k L[i][k] W[k] on the VALU with L through the scalar cache.
-struct GlobalW {
-  const float* ws;
-  uint64_t n;
-  __device__ __forceinline__ float operator()(int k, uint64_t path) const {
-    return path < n ? ws[(uint64_t)k * n + path] : 0.f;
-  }
-};
-
-template <bool kGreeks, bool kMfma>
-__global__ __launch_bounds__(kRegBlock) void basket_payoff_kernel(const BasketDev bd,
-                                                                  const float* __restrict__ ws,
-                                                                  double* __restrict__ partials) {
-  __shared__ double red[kRegBlock / 64][2 + 3 * kMaxAssets];
-  const uint32_t d = bd.d;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t wave_path0 = (uint64_t)blockIdx.x * kRegBlock + (threadIdx.x & ~63);
-  const GlobalW W{ws, bd.n_paths};
-  double psum = 0.0, psq = 0.0;
-  double gd = 0.0, gm = 0.0, gv = 0.0;  // VALU: lane a owns asset a's sums
-  double md[4] = {0, 0, 0, 0}, mm[4] = {0, 0, 0, 0}, mv[4] = {0, 0, 0, 0};  // MFMA
-  if constexpr (kMfma) {
-    basket_mfma_epilogue_w(bd, W, wave_path0, psum, psq, md, mm, mv, kGreeks);
-    // lanes l, l^16, l^32, l^48 hold the same four assets: fold them, then
-    // lane li (< 16) owns assets li, 16+li, 32+li, 48+li
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      md[nt] += __shfl_xor(md[nt], 16, 64);
-      md[nt] += __shfl_xor(md[nt], 32, 64);
-      mm[nt] += __shfl_xor(mm[nt], 16, 64);
-      mm[nt] += __shfl_xor(mm[nt], 32, 64);
-      mv[nt] += __shfl_xor(mv[nt], 16, 64);
-      mv[nt] += __shfl_xor(mv[nt], 32, 64);
-    }
-  } else {
-    const uint64_t p = wave_path0 + lane;
-    const bool valid = p < bd.n_paths;
-    double B = 0.0;
-#pragma unroll 1
-    for (uint32_t i = 0; i < d; ++i) {
-      float e = 0.f;
-      for (uint32_t k = 0; k <= i; ++k) e = fmaf(bd.Lf[i * 64 + k], W(k, p), e);
-      B += bd.w[i] * (bd.s0[i] * exp(bd.lmean[i] + bd.diff[i] * (double)e));
-    }
-    const bool itm = valid && B > bd.strike;
-    const double pay = itm ? B - bd.strike : 0.0;
-    psum += pay;
-    psq += pay * pay;
-    if constexpr (kGreeks) {
-#pragma unroll 1
-      for (uint32_t i = 0; i < d; ++i) {
-        double cd = 0.0, cm = 0.0, cv = 0.0;
-        if (itm) {
-          float e = 0.f;
-          for (uint32_t k = 0; k <= i; ++k) e = fmaf(bd.Lf[i * 64 + k], W(k, p), e);
-          const double sb = bd.w[i] * (bd.s0[i] * exp(bd.lmean[i] + bd.diff[i] * (double)e));
-          cd = sb / bd.s0[i];
-          cm = sb * bd.t;
-          cv = sb * (-bd.sigma[i] * bd.t + bd.sqrt_dt * (double)e);
-        }
-        cd = wave_sum_t(cd);
-        cm = wave_sum_t(cm);
-        cv = wave_sum_t(cv);
-        if ((uint32_t)lane == i) {
-          gd += cd;
-          gm += cm;
-          gv += cv;
-        }
-      }
     }
   }
   psum = wave_sum_t(psum);
@@ -459,60 +400,62 @@ __global__ __launch_bounds__(kRegBlock) void basket_payoff_kernel(const BasketDe
   }
 }
 
-template <bool kMfma>
-static int basket_reg_launch(const BasketDev& bd, const StreamDesc& sd, bool greeks,
-                             double* h_out, int na, hipStream_t st, const Err& err) {
-  const uint64_t nb = (bd.n_paths + kRegBlock - 1) / kRegBlock;  // phase 2: 1 path / thread
-  const uint64_t nb1 = (bd.n_paths + kSumsBlock - 1) / kSumsBlock;
-  if (nb > 0x7fffffffull) return err.set(STOCHA_EINVAL, "n_paths too large for one launch");
+template <bool kFp32>
+static void launch_sums(const BasketDev& bd, const StreamDesc& sd, uint64_t p0,
+                        typename ZigWave<kFp32>::Real* ws, hipStream_t st) {
+  const unsigned nb = (unsigned)((bd.n_paths + kSumsBlock - 1) / kSumsBlock);
+  const uint32_t nc = (bd.d + kChunk - 1) / kChunk;
+  if (nc <= 1)
+    basket_sums_kernel<kFp32, 1><<<nb, kSumsBlock, 0, st>>>(bd, sd, p0, ws);
+  else if (nc <= 2)
+    basket_sums_kernel<kFp32, 2><<<nb, kSumsBlock, 0, st>>>(bd, sd, p0, ws);
+  else if (nc <= 4)
+    basket_sums_kernel<kFp32, 4><<<nb, kSumsBlock, 0, st>>>(bd, sd, p0, ws);
+  else
+    basket_sums_kernel<kFp32, 8><<<nb, kSumsBlock, 0, st>>>(bd, sd, p0, ws);
+}
+
+// Both phases over slices of the paths; per-block partial sums of all
+// slices are reduced together at the end.
+template <bool kFp32, bool kMfma>
+static int basket_two_phase(const BasketDev& bd, const StreamDesc& sd, bool greeks,
+                            double* h_out, int na, hipStream_t st, const Err& err) {
+  using Real = typename ZigWave<kFp32>::Real;
+  const uint64_t slice = bd.n_paths < kSlicePaths ? bd.n_paths : kSlicePaths;
+  const uint64_t n_slices = (bd.n_paths + slice - 1) / slice;
+  uint64_t nbp = (slice + kRegBlock - 1) / kRegBlock;  // phase-2 blocks per slice
+  if (nbp > kPayoffBlocks) nbp = kPayoffBlocks;
+  const uint64_t n_part = n_slices * nbp;
+  if (n_part > 0x7fffffffull) return err.set(STOCHA_EINVAL, "n_paths too large for one call");
   Workspace* wsp = nullptr;
-  const size_t w_bytes = ((sizeof(float) * bd.d * bd.n_paths + 255) / 256) * 256;
-  const size_t part_bytes = sizeof(double) * nb * na;
+  const size_t w_bytes = ((sizeof(Real) * bd.d * slice + 255) / 256) * 256;
+  const size_t part_bytes = sizeof(double) * n_part * na;
   if (int rc = workspace(w_bytes + part_bytes + sizeof(double) * na, sizeof(double) * na, &wsp,
                          err))
     return rc;
-  float* ws = static_cast<float*>(wsp->dev);
+  Real* ws = static_cast<Real*>(wsp->dev);
   double* part = reinterpret_cast<double*>(static_cast<char*>(wsp->dev) + w_bytes);
-  double* out = part + nb * na;
+  double* out = part + n_part * na;
   KernelTimer timer(st);
   timer.start();
-  basket_sums_kernel<<<(unsigned)nb1, kSumsBlock, 0, st>>>(bd, sd, ws);
-  STOCHA_HIP(hipGetLastError());
-  if (greeks)
-    basket_payoff_kernel<true, kMfma><<<(unsigned)nb, kRegBlock, 0, st>>>(bd, ws, part);
-  else
-    basket_payoff_kernel<false, kMfma><<<(unsigned)nb, kRegBlock, 0, st>>>(bd, ws, part);
-  STOCHA_HIP(hipGetLastError());
-  timer.stop(2);
-  if (int rc = launch_reduce_partials(part, (int)nb, na, out, st, err)) return rc;
+  for (uint64_t s = 0; s < n_slices; ++s) {
+    BasketDev bs = bd;
+    const uint64_t p0 = s * slice;
+    bs.n_paths = bd.n_paths - p0 < slice ? bd.n_paths - p0 : slice;
+    launch_sums<kFp32>(bs, sd, p0, ws, st);
+    STOCHA_HIP(hipGetLastError());
+    double* ps = part + s * nbp * na;
+    if (greeks)
+      basket_payoff_kernel<Real, true, kMfma><<<(unsigned)nbp, kRegBlock, 0, st>>>(bs, ws, ps);
+    else
+      basket_payoff_kernel<Real, false, kMfma><<<(unsigned)nbp, kRegBlock, 0, st>>>(bs, ws, ps);
+    STOCHA_HIP(hipGetLastError());
+  }
+  timer.stop(2 * n_slices);
+  if (int rc = launch_reduce_partials(part, (int)n_part, na, out, st, err)) return rc;
   STOCHA_HIP(hipMemcpyAsync(wsp->host, out, sizeof(double) * na, hipMemcpyDeviceToHost, st));
   STOCHA_HIP(hipStreamSynchronize(st));
   for (int i = 0; i < na; ++i) h_out[i] = wsp->host[i];
-  return STOCHA_OK;
-}
-
-template <bool kFp32, int kBlock, bool kMfma>
-static int basket_launch(const BasketDev& bd, const StreamDesc& sd, bool greeks, double* h_out,
-                         int na, hipStream_t st, const Err& err) {
-  uint64_t nb = (bd.n_paths + kBlock - 1) / kBlock;
-  if (nb > 4096) nb = 4096;
-  DevBuf part, out;
-  if (int rc = part.alloc(sizeof(double) * nb * na, err)) return rc;
-  if (int rc = out.alloc(sizeof(double) * na, err)) return rc;
-  KernelTimer timer(st);
-  timer.start();
-  if (greeks)
-    basket_kernel<kFp32, kBlock, true, kMfma>
-        <<<(unsigned)nb, kBlock, 0, st>>>(bd, sd, part.as<double>());
-  else
-    basket_kernel<kFp32, kBlock, false, kMfma>
-        <<<(unsigned)nb, kBlock, 0, st>>>(bd, sd, part.as<double>());
-  STOCHA_HIP(hipGetLastError());
-  timer.stop(1);
-  if (int rc = launch_reduce_partials(part.as<double>(), (int)nb, na, out.as<double>(), st, err))
-    return rc;
-  STOCHA_HIP(hipMemcpyAsync(h_out, out.p, sizeof(double) * na, hipMemcpyDeviceToHost, st));
-  STOCHA_HIP(hipStreamSynchronize(st));
   return STOCHA_OK;
 }
 
@@ -564,10 +507,10 @@ int run_basket(const double* s0, const double* mu, const double* sigma, const do
   make_stream_desc(seed, (steps * d + 1024) * 2, &sd);  // multi_gbm.rs:40
   const int na = greeks ? 2 + 3 * (int)d : 2;
   if (precision == STOCHA_FP32) {
-    if (mfma) return basket_reg_launch<true>(bd, sd, greeks, h_sums, na, st, err);
-    return basket_reg_launch<false>(bd, sd, greeks, h_sums, na, st, err);
+    if (mfma) return basket_two_phase<true, true>(bd, sd, greeks, h_sums, na, st, err);
+    return basket_two_phase<true, false>(bd, sd, greeks, h_sums, na, st, err);
   }
-  return basket_launch<false, 64, false>(bd, sd, greeks, h_sums, na, st, err);
+  return basket_two_phase<false, false>(bd, sd, greeks, h_sums, na, st, err);
 }
 
 }  // namespace stocha

@@ -194,6 +194,63 @@ struct NoHook {
   __device__ __forceinline__ void operator()(uint32_t, const M&) const {}
 };
 
+// The stream increment with the two words that meet a carry-in (v_addc) in
+// VGPRs for the whole loop: an SGPR operand next to VCC would exceed gfx9's
+// one scalar operand per VALU instruction and cost a v_mov per draw.
+__device__ __forceinline__ U128 vgpr_increment(const U128 inc) {
+  U128 incv = inc;
+  uint32_t w1 = (uint32_t)(inc.lo >> 32), w2 = (uint32_t)inc.hi;
+  asm("" : "+v"(w1));
+  asm("" : "+v"(w2));
+  incv.lo = ((uint64_t)w1 << 32) | (uint32_t)inc.lo;
+  incv.hi = (inc.hi & 0xffffffff00000000ull) | w2;
+  return incv;
+}
+
+// One fill of the per-lane rings q (kSlots entries per lane, the lane's
+// `cnt` entries starting at `head`): draws until every lane holds at least
+// `need` entries.  Bottom-tested: one draw is always taken (by lanes with
+// room), which keeps the loop-carried state in place across the test.
+// Waiting lanes (their next draw goes to the slow block) are served once
+// Src::kDefer of them wait or one of them lacks entries.
+template <class Src, uint32_t kSlots, int kCols>
+__device__ __forceinline__ void queue_fill(Src& src, Pcg& g, const U128& incv,
+                                           const typename Src::Tables* zig,
+                                           typename Src::Rec (*q)[kCols], uint32_t head,
+                                           uint32_t& cnt, uint32_t need_n) {
+  static_assert((kSlots & (kSlots - 1)) == 0, "ring size");
+  constexpr uint32_t kCap = kSlots - 1;
+  using Rec = typename Src::Rec;
+  // lanes that still lack the entries asked for (a wave-uniform lane mask)
+  uint64_t need = lanes(cnt < need_n);
+  do {
+    // A lane whose next draw goes to the ziggurat's slow block holds it
+    // until the rule above serves the waiting lanes, so that one pass
+    // through the slow block serves many lanes (without the wait, 69% of a
+    // wave's draws would take it).  The lane's own draws are consumed in the
+    // same order either way.
+    uint64_t act = lanes(cnt < kCap);
+    if constexpr (Src::kDefer > 0) {
+      const uint64_t waiting = src.pending_mask();
+      // 32-bit popcounts: the scalar unit has no 64-bit ordered compare
+      const uint32_t n_wait = (uint32_t)__builtin_popcount((uint32_t)waiting) +
+                              (uint32_t)__builtin_popcount((uint32_t)(waiting >> 32));
+      const bool go = (n_wait >= Src::kDefer) | ((waiting & need) != 0);
+      act &= go ? ~0ull : ~waiting;
+    }
+    const uint64_t u = g.output();
+    const U128 sn = g.stepped(incv);
+    const bool a = lane_in(act);
+    g.s.lo = a ? sn.lo : g.s.lo;
+    g.s.hi = a ? sn.hi : g.s.hi;
+    Rec r;
+    const uint64_t got = src.feed(u, zig, r, act);
+    q[(head + cnt) & (kSlots - 1)][threadIdx.x] = r;
+    cnt += lane_in(got) ? 1u : 0u;
+    need = lanes(cnt < need_n);
+  } while (need != 0);
+}
+
 // The loop itself, on a caller-owned model (for models whose state the
 // caller reads afterwards).
 template <class Src, uint32_t kSlots, int kCols, class Model, class Hook = NoHook>
@@ -201,52 +258,11 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
                                           uint32_t steps, const typename Src::Tables* zig,
                                           typename Src::Rec (*q)[kCols], Hook hook = Hook()) {
   static_assert((kSlots & (kSlots - 1)) == 0 && kSlots > Src::kPerStep, "ring size");
-  constexpr uint32_t kCap = kSlots - 1;
   constexpr uint32_t kNeed = Src::kPerStep;
-  using Rec = typename Src::Rec;
-  // The two increment words that meet a carry-in (v_addc) live in VGPRs for
-  // the whole loop: an SGPR operand next to VCC would exceed gfx9's one
-  // scalar operand per VALU instruction and cost a v_mov per draw.
-  U128 incv = inc;
-  {
-    uint32_t w1 = (uint32_t)(inc.lo >> 32), w2 = (uint32_t)inc.hi;
-    asm("" : "+v"(w1));
-    asm("" : "+v"(w2));
-    incv.lo = ((uint64_t)w1 << 32) | (uint32_t)inc.lo;
-    incv.hi = (inc.hi & 0xffffffff00000000ull) | w2;
-  }
+  const U128 incv = vgpr_increment(inc);
   uint32_t head = 0, cnt = 0;
   for (uint32_t step = 0; step < steps; ++step) {
-    // lanes that still lack this step's input (a wave-uniform lane mask)
-    uint64_t need = lanes(cnt < kNeed);
-    // bottom-tested: one draw per step is always taken (by lanes with room),
-    // which keeps the loop-carried state in place across the test
-    do {
-      // A lane whose next draw goes to the ziggurat's slow block holds it
-      // until Src::kDefer lanes are waiting or one of the waiting lanes has run
-      // dry, so that one pass through the slow block serves many lanes
-      // (without the wait, 69% of a wave's draws would take it).  The lane's
-      // own draws are consumed in the same order either way.
-      uint64_t act = lanes(cnt < kCap);
-      if constexpr (Src::kDefer > 0) {
-        const uint64_t waiting = src.pending_mask();
-        // 32-bit popcounts: the scalar unit has no 64-bit ordered compare
-        const uint32_t n_wait = (uint32_t)__builtin_popcount((uint32_t)waiting) +
-                                (uint32_t)__builtin_popcount((uint32_t)(waiting >> 32));
-        const bool go = (n_wait >= Src::kDefer) | ((waiting & need) != 0);
-        act &= go ? ~0ull : ~waiting;
-      }
-      const uint64_t u = g.output();
-      const U128 sn = g.stepped(incv);
-      const bool a = lane_in(act);
-      g.s.lo = a ? sn.lo : g.s.lo;
-      g.s.hi = a ? sn.hi : g.s.hi;
-      Rec r;
-      const uint64_t got = src.feed(u, zig, r, act);
-      q[(head + cnt) & (kSlots - 1)][threadIdx.x] = r;
-      cnt += lane_in(got) ? 1u : 0u;
-      need = lanes(cnt < kNeed);
-    } while (need != 0);
+    queue_fill<Src, kSlots, kCols>(src, g, incv, zig, q, head, cnt, kNeed);
     Src::template apply<kSlots, kCols>(m, q, head);
     head = (head + kNeed) & (kSlots - 1);
     cnt -= kNeed;

@@ -44,7 +44,8 @@ def oracle_basket(orc, s0, mu, sigma, corr, w, k, r, t, steps, n, seed):
     return disc * pay.mean(), delta, dmu, vega
 
 
-@pytest.mark.parametrize("d", [1, 3, 8])
+# d = 9, 17, 33 end on a partial chunk of the draw kernel (chunks of 8 assets)
+@pytest.mark.parametrize("d", [1, 3, 8, 9, 17, 33, 64])
 def test_basket_fp64_matches_oracle(gpu, orc, d):
     s0, mu, sigma, corr, w, k = config5_inputs(d, seed=d)
     kw = dict(r=0.03, t=1.0, steps=40, n_paths=3000, seed=11)
@@ -55,6 +56,22 @@ def test_basket_fp64_matches_oracle(gpu, orc, d):
     np.testing.assert_allclose(res["delta"], dl, rtol=1e-10, atol=1e-14)
     np.testing.assert_allclose(res["dmu"], dm, rtol=1e-10, atol=1e-14)
     np.testing.assert_allclose(res["vega"], vg, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_basket_several_path_slices(gpu, orc, precision):
+    """More paths than one slice of the two-kernel form (2^22 paths): the
+    second slice continues the streams at path 2^22 and its partial sums join
+    the first slice's."""
+    s0, mu, sigma, corr, w, k = config5_inputs(2, seed=21)
+    kw = dict(r=0.03, t=1.0, steps=3, n_paths=(1 << 22) + 1000, seed=13)
+    res = gpu.basket_price(s0, mu, sigma, corr, w, k, precision=precision, greeks=True, **kw)
+    p, dl, dm, vg = oracle_basket(orc, s0, mu, sigma, corr, w, k, kw["r"], kw["t"], kw["steps"],
+                                  kw["n_paths"], kw["seed"])
+    tol = 1e-11 if precision == 64 else 1e-4
+    assert res["price"] == pytest.approx(p, rel=tol)
+    np.testing.assert_allclose(res["delta"], dl, rtol=100 * tol)
+    np.testing.assert_allclose(res["vega"], vg, rtol=1000 * tol)
 
 
 def test_basket_fp32_matches_oracle(gpu, orc):
