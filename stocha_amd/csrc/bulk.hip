@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kBulkBlock) void gauss_copula_kernel(const double* 
 //              (the normals and sqrt(nu / chi2));
 //   6. cdf:    u_ij = t_cdf((L z_i)_j sqrt(nu / chi2_i), nu).
 // A sample longer than kTChunk draws (d above ~50) overflows the maps; then
-// one thread walks the starts with len(p) instead (same values, slower).
+// the starts come from pointer doubling over len(p) instead (same values).
 constexpr int kTChunk = 64;
 constexpr uint32_t kLenBad = 0xffffu;
 // largest dimension: a sample takes ~2.1 + 1.03 d draws, far below kLenBad
@@ -662,19 +662,82 @@ __global__ __launch_bounds__(kBulkBlock) void t_starts_kernel(const uint16_t* __
   }
 }
 
-// one thread walks the starts (maps overflowed: samples above kTChunk draws);
-// *found = the samples whose starts lie below n_pos
-__global__ void t_walk_kernel(const uint16_t* __restrict__ len, const uint64_t n_pos,
-                              const uint64_t n_samples, uint64_t* __restrict__ starts,
-                              uint64_t* __restrict__ found) {
-  uint64_t p = 0, i = 0;
-  for (; i < n_samples && p < n_pos; ++i) {
-    const uint32_t l = len[p];
-    if (l == kLenBad) break;
-    starts[i] = p;
-    p += l;
+// Samples above kTChunk draws (the maps overflow): the starts by pointer
+// doubling.  J0(p) = p + len(p) is the next start after a start at p (kNoPos
+// where len is bad or the next start lies at or past n_pos); doubling a table
+// (J(J(p))) six times gives J6 = J0^64, six more J12 = J0^4096.  One thread
+// walks J12 from 0 (every 4096th start), one thread per such start walks J6
+// (every 64th), one per those walks J0 and writes the starts; a walk that
+// meets kNoPos records where the chain ends (*found, atomicMin).
+constexpr uint32_t kNoPos = 0xffffffffu;
+constexpr int kJumpLog = 6;  // J6 = J0^(2^kJumpLog)
+constexpr uint64_t kJumpSpan = 1ull << kJumpLog;
+
+__global__ __launch_bounds__(kBulkBlock) void t_next_kernel(const uint16_t* __restrict__ len,
+                                                            const uint64_t n_pos,
+                                                            uint32_t* __restrict__ J) {
+  const uint64_t p = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (p >= n_pos) return;
+  const uint32_t l = len[p];
+  J[p] = (l == kLenBad || p + l >= n_pos) ? kNoPos : (uint32_t)(p + l);
+}
+
+__global__ __launch_bounds__(kBulkBlock) void t_double_kernel(const uint32_t* __restrict__ Jin,
+                                                              const uint64_t n_pos,
+                                                              uint32_t* __restrict__ Jout) {
+  const uint64_t p = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (p >= n_pos) return;
+  const uint32_t q = Jin[p];
+  Jout[p] = q == kNoPos ? kNoPos : Jin[q];
+}
+
+// level 2: one thread, every (kJumpSpan^2)-th start into top[] (kNoPos past the end)
+__global__ void t_walk_top_kernel(const uint32_t* __restrict__ J12, const uint64_t n_top,
+                                  uint32_t* __restrict__ top) {
+  uint32_t p = 0;
+  for (uint64_t i = 0; i < n_top; ++i) {
+    top[i] = p;
+    if (p != kNoPos) p = J12[p];
   }
-  *found = i;
+}
+
+// level 1: thread i walks J6 from top[i]: mid[i * kJumpSpan + k]
+__global__ __launch_bounds__(kBulkBlock) void t_walk_mid_kernel(const uint32_t* __restrict__ top,
+                                                                const uint64_t n_top,
+                                                                const uint32_t* __restrict__ J6,
+                                                                const uint64_t n_mid,
+                                                                uint32_t* __restrict__ mid) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (i >= n_top) return;
+  uint32_t p = top[i];
+  for (uint64_t k = 0; k < kJumpSpan; ++k) {
+    const uint64_t m = i * kJumpSpan + k;
+    if (m >= n_mid) return;
+    mid[m] = p;
+    if (p != kNoPos) p = J6[p];
+  }
+}
+
+// level 0: thread m walks J0 from mid[m] and writes starts m*kJumpSpan + k;
+// the first missing start (past n_pos, or a sample that runs past the decoded
+// draws) bounds *found
+__global__ __launch_bounds__(kBulkBlock) void t_walk_low_kernel(
+    const uint32_t* __restrict__ mid, const uint64_t n_mid, const uint32_t* __restrict__ J0,
+    const uint16_t* __restrict__ len, const uint64_t n_samples, uint64_t* __restrict__ starts,
+    unsigned long long* __restrict__ found) {
+  const uint64_t m = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
+  if (m >= n_mid) return;
+  uint32_t p = mid[m];
+  for (uint64_t k = 0; k < kJumpSpan; ++k) {
+    const uint64_t s = m * kJumpSpan + k;
+    if (s >= n_samples) return;
+    if (p == kNoPos || len[p] == kLenBad) {
+      atomicMin(found, (unsigned long long)s);
+      return;
+    }
+    starts[s] = p;
+    p = J0[p];
+  }
 }
 
 __global__ __launch_bounds__(kBulkBlock) void t_emit_kernel(const uint64_t* __restrict__ U,
@@ -957,20 +1020,32 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
   const bool host = where != STOCHA_DEVICE;
   // draws per sample ~2.1 + 1.03 d; positions with margin, grown if short
   uint64_t n_chunks = (uint64_t)((double)n_samples * (2.4 + 1.08 * (double)d)) / kTChunk + 4;
+  // the doubling tables of the long-sample walk: a sample takes at least d + 2
+  // draws, so below d = 16 it overflows a 64-draw chunk only after ~25
+  // rejected gamma attempts; the tables are then added on the retry
+  bool walk_tables = d >= 16;
   for (int attempt = 0; attempt < 8; ++attempt) {
     const uint64_t n_pos = n_chunks * kTChunk;
     const uint64_t n_tot = n_pos + 16 * (d + 8) + 1024;  // the last samples' draws
     if (n_tot > (1ull << 40)) return err.set(STOCHA_EINVAL, "size too large");
+    if (walk_tables && n_pos >= (uint64_t)kNoPos)
+      return err.set(STOCHA_EINVAL, "size too large");
     const size_t nc = n_chunks;
     const uint64_t n_blk = (nc + kBulkBlock - 1) / kBulkBlock;
     const size_t tmp = 8 * scan_tmp_words(nc);
+    const uint64_t n_mid = (n_samples + kJumpSpan - 1) / kJumpSpan;
+    const uint64_t n_top = (n_mid + kJumpSpan - 1) / kJumpSpan;
+    const size_t jt = walk_tables ? 4 * n_pos : 0;  // each of J0, J6, and two doubling buffers
     const size_t sz[] = {8 * n_tot,     2 * n_pos,     sizeof(TMap) * nc, 0,
                          kTChunk * nc,  nc,            8 * nc,            8 * nc,
                          256,           tmp,           8 * n_samples,     8 * n,
                          8 * n_samples, 8 * d * d,     host ? 8 * n : 0,
-                         sizeof(TMap) * n_blk, n_blk};
-    size_t o[17], total = 0;
-    for (int i = 0; i < 17; ++i) {
+                         sizeof(TMap) * n_blk, n_blk,
+                         jt,            jt,            jt,                jt,
+                         walk_tables ? 4 * n_mid : 0, walk_tables ? 4 * n_top : 0};
+    constexpr int kRegions = 23;
+    size_t o[kRegions], total = 0;
+    for (int i = 0; i < kRegions; ++i) {
       o[i] = total;
       total += al256(sz[i]);
     }
@@ -985,7 +1060,7 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     uint64_t* cnt_true = reinterpret_cast<uint64_t*>(b + o[6]);
     uint64_t* off = reinterpret_cast<uint64_t*>(b + o[7]);
     unsigned* flag = reinterpret_cast<unsigned*>(b + o[8]);
-    uint64_t* found = reinterpret_cast<uint64_t*>(b + o[8] + 64);
+    unsigned long long* found = reinterpret_cast<unsigned long long*>(b + o[8] + 64);
     void* scan_tmp = b + o[9];
     uint64_t* starts = reinterpret_cast<uint64_t*>(b + o[10]);
     double* Z = reinterpret_cast<double*>(b + o[11]);
@@ -994,6 +1069,12 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     double* dout = host ? reinterpret_cast<double*>(b + o[14]) : out;
     TMap* agg = reinterpret_cast<TMap*>(b + o[15]);
     uint8_t* xin = reinterpret_cast<uint8_t*>(b + o[16]);
+    uint32_t* J0 = reinterpret_cast<uint32_t*>(b + o[17]);
+    uint32_t* J6 = reinterpret_cast<uint32_t*>(b + o[18]);
+    uint32_t* T1 = reinterpret_cast<uint32_t*>(b + o[19]);
+    uint32_t* T2 = reinterpret_cast<uint32_t*>(b + o[20]);
+    uint32_t* mid = reinterpret_cast<uint32_t*>(b + o[21]);
+    uint32_t* top = reinterpret_cast<uint32_t*>(b + o[22]);
 
     const auto blocks = [](uint64_t items, uint64_t per) { return (unsigned)((items + per - 1) / per); };
     KernelTimer timer(st);
@@ -1017,10 +1098,31 @@ int stocha_student_t_copula(const double* corr, uint64_t rows, uint64_t cols, do
     STOCHA_HIP(hipMemcpyAsync(&last_off, off + nc - 1, 8, hipMemcpyDeviceToHost, st));
     STOCHA_HIP(hipMemcpyAsync(&last_cnt, cnt_true + nc - 1, 8, hipMemcpyDeviceToHost, st));
     STOCHA_HIP(hipStreamSynchronize(st));
-    if (h_flag & 1u) {  // a sample above kTChunk draws: walk the starts on one thread
-      t_walk_kernel<<<1, 1, 0, st>>>(len, n_pos, n_samples, starts, found);
+    if (h_flag & 1u) {  // a sample above kTChunk draws: the starts by pointer doubling
+      if (!walk_tables) {
+        walk_tables = true;
+        continue;
+      }
+      const unsigned pb = blocks(n_pos, kBulkBlock);
+      t_next_kernel<<<pb, kBulkBlock, 0, st>>>(len, n_pos, J0);
+      // J0 -> T1 -> T2 -> T1 -> T2 -> T1 -> J6 (x64), then J6 -> T1 -> T2 -> T1
+      // -> T2 -> T1 -> T2 (x4096): J12 ends in T2
+      const uint32_t* src = J0;
+      for (int r = 0; r < 2 * kJumpLog; ++r) {
+        uint32_t* dst = r == kJumpLog - 1 ? J6 : (r % 2 == 0 ? T1 : T2);
+        t_double_kernel<<<pb, kBulkBlock, 0, st>>>(src, n_pos, dst);
+        src = dst;
+      }
+      const unsigned long long n_all = n_samples;
+      STOCHA_HIP(hipMemcpyAsync(found, &n_all, 8, hipMemcpyHostToDevice, st));
+      t_walk_top_kernel<<<1, 1, 0, st>>>(T2, n_top, top);
+      t_walk_mid_kernel<<<blocks(n_top, kBulkBlock), kBulkBlock, 0, st>>>(top, n_top, J6, n_mid,
+                                                                          mid);
+      t_walk_low_kernel<<<blocks(n_mid, kBulkBlock), kBulkBlock, 0, st>>>(mid, n_mid, J0, len,
+                                                                          n_samples, starts,
+                                                                          found);
       STOCHA_HIP(hipGetLastError());
-      uint64_t h_found = 0;
+      unsigned long long h_found = 0;
       STOCHA_HIP(hipMemcpyAsync(&h_found, found, 8, hipMemcpyDeviceToHost, st));
       STOCHA_HIP(hipStreamSynchronize(st));
       if (h_found < n_samples) {

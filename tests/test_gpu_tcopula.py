@@ -7,8 +7,8 @@ shifts every later sample, so agreement on all rows at once checks the
 decoding end to end; the values then differ from the oracle only by the
 device's log/exp in the t CDF (1e-15 relative; the continued fraction stops
 at 3e-7, so a stop one step apart would move a value by ~1e-7: allowed for
-no element here).  Also the fallback for samples longer than a chunk's 64
-draws (d = 60), empty requests, and the reference's own properties
+no element here).  Also the pointer-doubling path for samples longer than a
+chunk's 64 draws (d >= 60), empty requests, and the reference's own properties
 (tests/test_risk.py:95-136) at a size the oracle does not run.
 """
 import numpy as np
@@ -39,12 +39,30 @@ def test_matches_oracle(gpu, orc, d, n, nu, seed):
     np.testing.assert_allclose(dev, ref, rtol=0, atol=1e-12)
 
 
-def test_long_samples_take_the_walk(gpu, orc):
-    # d = 60: every sample takes > 64 draws, beyond the chunk maps
-    corr = _corr(60, 1)
-    dev = gpu.student_t_copula(corr, nu=6.0, n_samples=300, seed=9)
-    ref = orc.student_t_copula(corr, 6.0, 300, seed=9)
+# d >= 60: every sample takes > 64 draws, beyond the chunk maps, so the
+# starts come from pointer doubling (jumps of 64 and 4096 samples): 300
+# samples use one top-level jump chain entry, 9000 three, and 100,000 at
+# d = 64 is the size where a serial walk of the starts took ~0.1 s
+@pytest.mark.parametrize("d,n,nu,seed", [(60, 300, 6.0, 9), (61, 9_000, 4.0, 2),
+                                         (200, 700, 7.5, 4)])
+def test_long_samples_take_pointer_doubling(gpu, orc, d, n, nu, seed):
+    corr = _corr(d, seed)
+    dev = gpu.student_t_copula(corr, nu=nu, n_samples=n, seed=seed)
+    ref = orc.student_t_copula(corr, nu, n, seed=seed)
     np.testing.assert_allclose(dev, ref, rtol=0, atol=1e-12)
+
+
+def test_long_samples_d64_100k_timed(gpu, orc):
+    import time
+    corr = _corr(64, 12)
+    gpu.student_t_copula(corr, nu=5.0, n_samples=1000, seed=1)  # workspace warm-up
+    t0 = time.perf_counter()
+    dev = gpu.student_t_copula(corr, nu=5.0, n_samples=100_000, seed=1)
+    dt = time.perf_counter() - t0
+    ref = orc.student_t_copula(corr, 5.0, 100_000, seed=1)
+    np.testing.assert_allclose(dev, ref, rtol=0, atol=1e-12)
+    print(f"student_t_copula d=64 100k: {dt * 1e3:.2f} ms")
+    assert dt < 0.05, dt
 
 
 def test_empty(gpu):
