@@ -10,6 +10,15 @@
 
 namespace stocha {
 
+// Block barrier ordering LDS only: unlike __syncthreads it does not wait for
+// this wave's outstanding global loads and stores, so memory operations
+// issued earlier stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Step-input sources for queue_path.  A source turns the lanes' draws, one
 // per lane per call, into queue entries; Src::kPerStep entries make one
 // step's input.  Sources are wave-collective: feed(u, z, rec, act) is called

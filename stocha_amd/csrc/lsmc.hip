@@ -580,15 +580,6 @@ __device__ __forceinline__ double wave_reduce_scatter16(const double (&v)[16]) {
   return r;
 }
 
-// Block barrier ordering LDS only: unlike __syncthreads it does not wait for
-// this wave's outstanding global loads, so loads issued earlier stay in
-// flight across it.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Block sums of acc[0..na) over the first kWaves waves (the other waves hold
 // nothing): per-wave reduce-scatter, then the waves in order; thread a < na
 // returns slot a's total (red is free again on return).  Called by every

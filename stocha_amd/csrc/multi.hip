@@ -85,8 +85,9 @@ struct MultiModel {
     // invariant before every step: the step's d values fit (tpos + d <= tcap)
     if (kTiled && tpos + a.d > tcap) flush();
   }
+  // (LDS-only barriers: the row stores stay in flight across them)
   __device__ __forceinline__ void flush() {
-    __syncthreads();
+    lds_barrier();
     const uint64_t row0 = (uint64_t)blockIdx.x * kMultiBlock;
     const uint64_t left = a.n_rows - row0;
     const int rows_here = left < (uint64_t)kMultiBlock ? (int)left : kMultiBlock;
@@ -97,7 +98,7 @@ struct MultiModel {
       const int c = e - r * (int)tpos;
       a.out[(row0 + r) * width + col0 + c] = tile[c][r];
     }
-    __syncthreads();
+    lds_barrier();
     col0 += tpos;
     tpos = 0;
   }

@@ -1,5 +1,5 @@
 // paths.hip -- path-storing FP64 steppers (the drop-in gbm / heston /
-// merton_jump_diffusion outputs).
+// merton_jump_diffusion / hull_white outputs).
 //
 // One thread owns one output row (one path, or its antithetic mirror) and
 // steps it with the reference's arithmetic (models.hpp) on the queue schedule
@@ -24,8 +24,9 @@ struct TileHook {
   double (*tile)[kPathBlock + 1];
   double* out;
   uint64_t row0, width, rows_here;
+  // (LDS-only barriers: the row stores stay in flight across them)
   __device__ __forceinline__ void flush(uint64_t c0, int cols) const {
-    __syncthreads();
+    lds_barrier();
     // element e of the tile is (row e / cols, col e % cols)
     const int n_el = (int)rows_here * cols;
     for (int e = threadIdx.x; e < n_el; e += kPathBlock) {
@@ -33,7 +34,7 @@ struct TileHook {
       const int c = e - r * cols;
       out[(row0 + r) * width + c0 + c] = tile[c][r];
     }
-    __syncthreads();
+    lds_barrier();
   }
   template <class M>
   __device__ __forceinline__ void operator()(uint32_t step, const M& m) const {
