@@ -216,6 +216,15 @@ __device__ __forceinline__ U128 vgpr_increment(const U128 inc) {
   return incv;
 }
 
+// c + 1 on the lanes of mask m, c elsewhere: one add with the lane mask as
+// the carry-in (a select of 0/1 and an add otherwise).
+__device__ __forceinline__ uint32_t add_lanes(uint32_t c, uint64_t m) {
+  uint32_t r;
+  uint64_t carry;
+  asm("v_addc_co_u32_e64 %0, %1, 0, %2, %3" : "=v"(r), "=s"(carry) : "v"(c), "s"(m));
+  return r;
+}
+
 // One fill of the per-lane rings q (kSlots entries per lane, the lane's
 // `cnt` entries starting at `head`): draws until every lane holds at least
 // `need` entries.  Bottom-tested: one draw is always taken (by lanes with
@@ -255,7 +264,7 @@ __device__ __forceinline__ void queue_fill(Src& src, Pcg& g, const U128& incv,
     Rec r;
     const uint64_t got = src.feed(u, zig, r, act);
     q[(head + cnt) & (kSlots - 1)][threadIdx.x] = r;
-    cnt += lane_in(got) ? 1u : 0u;
+    cnt = add_lanes(cnt, got);
     need = lanes(cnt < need_n);
   } while (need != 0);
 }

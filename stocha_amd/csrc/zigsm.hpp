@@ -130,8 +130,8 @@ struct ZigFeed {
 // per lane -- so every state transition is scalar-unit work and the VALU
 // only computes the draw's values and compares (whose ballots are free: a
 // compare's result already is a lane mask).  The rarely needed work (a tied
-// fast test, a wedge uniform, the second draw of a tail pair) runs under
-// uniform branches, each only when some lane needs it, with its outcome
+// fast test, the draw after a missed attempt, a tail pair's draws) runs
+// under uniform branches, each only when some lane needs it, with its outcome
 // ballotted back into the masks.  Wave-collective: every lane of the
 // wavefront calls feed together; `act` names the lanes that consume a draw.
 // A lane's draws, values and accept/reject decisions are exactly ZigFeed's.
@@ -144,13 +144,17 @@ template <bool kFp32>
 struct ZigWave {
   using Real = typename std::conditional<kFp32, float, double>::type;
   using Tables = typename std::conditional<kFp32, ZigLds32, ZigLds>::type;
-  uint64_t wedge = 0;  // lanes whose next draw is the wedge uniform of attempt ua
-  uint64_t tail1 = 0;  // lanes whose next draw is the first of a tail pair
-  uint64_t tail2 = 0;  // lanes whose next draw is the second of the pair starting at ua
-  uint64_t neg = 0;    // sign of the layer-0 attempt that led each tail
-  uint64_t ua = 0;     // per lane: the missed attempt (wedge) or the pair's first draw
+  // The states after a missed attempt are told apart only when the lane is
+  // served (in the slow block): the fast part neither tests the layer of a
+  // miss nor keeps its sign.
+  uint64_t missed = 0;  // lanes whose next draw follows the missed attempt ua:
+                        // the wedge uniform (layer > 0) or a tail pair's first
+  uint64_t tail1 = 0;   // lanes whose next draw starts a new tail pair
+  uint64_t tail2 = 0;   // lanes whose next draw is the second of the pair at ua
+  uint64_t neg = 0;     // sign of the layer-0 attempt that led each tail
+  uint64_t ua = 0;      // per lane: the missed attempt or the pair's first draw
 
-  __device__ __forceinline__ uint64_t pending_mask() const { return wedge | tail2; }
+  __device__ __forceinline__ uint64_t pending_mask() const { return missed | tail1 | tail2; }
 
   // Consumes draw u on the lanes of `act`; returns the lanes whose draw
   // completed a normal (in out).  `out` is written on every lane.
@@ -159,71 +163,42 @@ struct ZigWave {
     const uint32_t i = (uint32_t)u & 0x1ffu;
     const uint32_t hi = (uint32_t)(u >> 32);
     uint32_t th;
-    uint64_t sg;
     if constexpr (kFp32) {
       const ZigFast32 f = z->fast32[i];
       th = f.th;
-      out = hi32_to_f32(u) * f.xs;
-      sg = lanes(f.xs < 0.f);  // the table folds the sign bit (bit 8) into X
+      out = hi32_to_f32(u) * f.xs;  // the table folds the sign bit (bit 8) into X
     } else {
       const ZigFast f = z->fast[i];
       th = f.th;
       out = ((double)(u >> 11) * 0x1p-53) * f.x;
-      sg = lanes(f.x < 0.0);
     }
-    const uint64_t lt = lanes(hi < th), eq = lanes(hi == th), l0 = lanes((i & 0xffu) == 0);
-    const uint64_t at = act & ~(wedge | tail1 | tail2);
+    const uint64_t lt = lanes(hi < th), eq = lanes(hi == th);
+    const uint64_t held = missed | tail1 | tail2;
+    const uint64_t at = act & ~held;
     const uint64_t take = at & lt;
     // a tie on the high word (one draw in 2^32) is booked as a miss and
-    // settled with the full threshold below
+    // settled with the full threshold in the slow block
     const uint64_t tie = at & eq, miss = at & ~lt;
-    const uint64_t wl = act & wedge, t2 = act & tail2, t1 = act & tail1;
-    ua = lane_in(miss | t1) ? u : ua;
-    neg = (neg & ~miss) | (miss & sg);
-    wedge = (wedge & ~act) | (miss & ~l0);
-    tail1 = (tail1 & ~act) | (miss & l0);
-    tail2 = (tail2 & ~act) | t1;
+    const uint64_t served = act & held;  // their states are updated in the slow block
+    ua = lane_in(miss) ? u : ua;
+    missed |= miss;
     uint64_t got = take;
-    if ((tie | wl | t2) != 0) got |= slow(u, z, out, tie, wl, t2);
+    if ((tie | served) != 0) got |= slow(u, z, out, tie, served);
     return got;
   }
 
   __device__ __forceinline__ uint64_t slow(uint64_t u, const Tables* __restrict__ z, Real& out,
-                                           uint64_t tie, uint64_t wl, uint64_t t2) {
+                                           uint64_t tie, uint64_t served) {
     uint64_t got = 0;
+    // (this draw's misses, ties included, are not among the served lanes)
+    const uint64_t ml = served & missed, t1 = served & tail1, t2 = served & tail2;
+    missed &= ~ml;
+    tail1 &= ~t1;
+    tail2 &= ~t2;
     if (tie != 0) {  // exact fast-path test (normal.rs:26-29); out holds x
       const uint64_t ok = tie & lanes((u >> 11) < kZigT[u & 0xffu]);
-      wedge &= ~ok;
-      tail1 &= ~ok;
+      missed &= ~ok;
       got |= ok;
-    }
-    if (wl != 0) {  // wedge (normal.rs:36-42): u is the uniform, ua the attempt
-      // (every lane evaluates; indices are masked into the tables)
-      const uint32_t ia = (uint32_t)ua & 0x1ffu, l = ia & 0xffu;
-      Real xw;
-      uint64_t acc;
-      if constexpr (kFp32) {
-        xw = hi32_to_f32(ua) * z->fast32[ia].xs;  // the attempt's value
-        const float2 yd = z->ydy32[l];
-        const float y = fmaf(top24_to_f32(u) * 0x1p-24f, yd.y, yd.x);
-        const float e = __builtin_amdgcn_exp2f(-0.72134752044448170f * xw * xw);
-        const float diff = y - e;
-        acc = lanes(diff < 0.f);
-        const uint64_t close = wl & ~lanes(fabsf(diff) > 4e-6f);
-        if (close != 0) {  // inside the guard band: the FP64 test decides
-          const double x64 = ((double)(ua >> 11) * 0x1p-53) * kZigX[l];
-          const double y64 =
-              kZigY[l] + ((double)(u >> 11) * 0x1p-53) * (kZigY[l + 1] - kZigY[l]);
-          acc = (acc & ~close) | (close & lanes(y64 < exp(-0.5 * x64 * x64)));
-        }
-      } else {
-        xw = ((double)(ua >> 11) * 0x1p-53) * z->fast[ia].x;
-        const double y0 = z->y[l], y1 = z->y[l + 1];
-        const double y = y0 + ((double)(u >> 11) * 0x1p-53) * (y1 - y0);
-        acc = lanes(y < exp(-0.5 * xw * xw));
-      }
-      out = lane_in(wl) ? xw : out;  // (feed has already left the wedge state)
-      got |= wl & acc;
     }
     if (t2 != 0) {  // tail (normal.rs:54-67): u is the pair's second draw
       Real t;
@@ -237,6 +212,48 @@ struct ZigWave {
       tail1 |= t2 & ~okm;  // rejected: the next draw starts a new pair
       out = lane_in(okm) ? (lane_in(neg) ? -t : t) : out;
       got |= okm;
+    }
+    if (ml != 0) {
+      const uint32_t ia = (uint32_t)ua & 0x1ffu, l = ia & 0xffu;
+      const uint64_t l0 = ml & lanes(l == 0u);
+      const uint64_t wl = ml & ~l0;
+      if (wl != 0) {  // wedge (normal.rs:36-42): u is the uniform, ua the attempt
+        // (every lane evaluates; indices are masked into the tables)
+        Real xw;
+        uint64_t acc;
+        if constexpr (kFp32) {
+          xw = hi32_to_f32(ua) * z->fast32[ia].xs;  // the attempt's value
+          const float2 yd = z->ydy32[l];
+          const float y = fmaf(top24_to_f32(u) * 0x1p-24f, yd.y, yd.x);
+          const float e = __builtin_amdgcn_exp2f(-0.72134752044448170f * xw * xw);
+          const float diff = y - e;
+          acc = lanes(diff < 0.f);
+          const uint64_t close = wl & ~lanes(fabsf(diff) > 4e-6f);
+          if (close != 0) {  // inside the guard band: the FP64 test decides
+            const double x64 = ((double)(ua >> 11) * 0x1p-53) * kZigX[l];
+            const double y64 =
+                kZigY[l] + ((double)(u >> 11) * 0x1p-53) * (kZigY[l + 1] - kZigY[l]);
+            acc = (acc & ~close) | (close & lanes(y64 < exp(-0.5 * x64 * x64)));
+          }
+        } else {
+          xw = ((double)(ua >> 11) * 0x1p-53) * z->fast[ia].x;
+          const double y0 = z->y[l], y1 = z->y[l + 1];
+          const double y = y0 + ((double)(u >> 11) * 0x1p-53) * (y1 - y0);
+          acc = lanes(y < exp(-0.5 * xw * xw));
+        }
+        out = lane_in(wl) ? xw : out;
+        got |= wl & acc;
+      }
+      if (l0 != 0) {  // layer 0 (normal.rs:54): u is the first of a tail pair;
+                      // the attempt's sign (bit 8) signs the tail value
+        neg = (neg & ~l0) | (l0 & lanes((ia & 0x100u) != 0u));
+        ua = lane_in(l0) ? u : ua;
+        tail2 |= l0;
+      }
+    }
+    if (t1 != 0) {  // the first draw of a new pair
+      ua = lane_in(t1) ? u : ua;
+      tail2 |= t1;
     }
     return got;
   }
