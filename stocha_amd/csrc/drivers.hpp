@@ -37,7 +37,7 @@ struct NormalSrc {
   using Rec = Real;
   static constexpr uint32_t kPerStep = kPer;
   static constexpr uint32_t kDefer = 8;  // slow-block deferral (queue_run)
-  static constexpr int kWaves = kFp32 ? 6 : 4;  // occupancy target of the fused kernel
+  static constexpr int kWaves = kFp32 ? 7 : 4;  // occupancy target of the fused kernel
   ZigWave<kFp32> zf;
   template <class C>
   __device__ __forceinline__ explicit NormalSrc(const C&) {}

@@ -132,14 +132,18 @@ __device__ __forceinline__ void finish_block_sums(double s, int na, double (*red
                                                   double* partials,
                                                   unsigned* __restrict__ counter,
                                                   double* __restrict__ out) {
-  __shared__ bool last;
   if (threadIdx.x < na) put_slot(&partials[(uint64_t)blockIdx.x * na + threadIdx.x], s);
   __syncthreads();
+  // the election's result goes through red[0][0] (red is free here: s is in
+  // registers), not a flag of its own -- 8 more bytes of LDS would cost the
+  // 23,040-byte fused kernel its seventh block per CU
   if (threadIdx.x == 0)
-    last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-           gridDim.x - 1;
+    red[0][0] = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1
+                    ? 1.0
+                    : 0.0;
   __syncthreads();
-  if (!last) return;
+  if (red[0][0] == 0.0) return;
   double tot[kMaxAcc];
 #pragma unroll
   for (int i = 0; i < kMaxAcc; ++i) tot[i] = 0.0;
