@@ -121,8 +121,9 @@ __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
 // so the step arithmetic never runs with half the wave masked.  Simulated on
 // the draw-count distribution of the ziggurat, a ring of 8 Heston normals
 // costs ~2.14 draws per step against the 2.06 minimum.
-// Occupancy target per source (Src::kWaves): the FP32 GBM/Heston kernels fit
-// 6 waves per SIMD (<= 80 VGPRs, 23 KB of LDS per block) without spilling;
+// Occupancy target per source (Src::kWaves): the FP32 GBM/Heston kernels run
+// 7 waves per SIMD (<= 72 VGPRs, with three dwords spilled outside the path
+// loop; 23,040 bytes of LDS per block, which seven blocks fit in 160 KB);
 // the others need more registers and run 4.
 template <class Src, class Model, class Const, class Payoff, int kQueue>
 __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
