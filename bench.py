@@ -19,6 +19,8 @@ and one core on a 1/16 sample, and LSMC on a 262,144-path sample plus one
 
 Under rocprofv3, run with STOCHA_AMD_TORCH=0 so that the process maps one
 HIP runtime (the library's, the profiler's version) instead of torch's too.
+The default --warmup 30 covers the GPU clock's ramp (the headline kernel
+runs 1.13 ms per call at first, 0.99 ms from about the 25th call on).
 """
 from __future__ import annotations
 
@@ -177,8 +179,10 @@ def cpu_baselines(S, out, r32, r64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # The GPU clock ramps up over the first ~25 calls of the headline kernel
+    # (1.13 -> 0.99 ms per call); 30 warm-up calls time the steady state.
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--calib-copy", action="store_true",
