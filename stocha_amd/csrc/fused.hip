@@ -121,10 +121,10 @@ __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
 // so the step arithmetic never runs with half the wave masked.  Simulated on
 // the draw-count distribution of the ziggurat, a ring of 8 Heston normals
 // costs ~2.14 draws per step against the 2.06 minimum.
-// Occupancy target per source (Src::kWaves): the FP32 GBM/Heston kernels run
-// 7 waves per SIMD (<= 72 VGPRs, with three dwords spilled outside the path
-// loop; 23,040 bytes of LDS per block, which seven blocks fit in 160 KB);
-// the others need more registers and run 4.
+// Occupancy target per source (Src::kWaves): the FP32 GBM/Heston/Merton
+// kernels run 7 waves per SIMD (<= 72 VGPRs, with three dwords spilled
+// outside the path loop; 23,040 bytes of LDS per block, which seven blocks
+// fit in 160 KB); the FP64 ones need more registers and run 4.
 template <class Src, class Model, class Const, class Payoff, int kQueue>
 __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
     Src::kWaves))) void terminal_feed_kernel(
