@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kBulkBlock) void seq_uniform_kernel(const StreamDes
 }
 
 // One normal from stream position p (g at p): the value; p and g move past it.
-__device__ __forceinline__ double decode_one(Pcg& g, const U128 inc, const ZigLds* z,
+__device__ __forceinline__ double decode_one(Pcg& g, const U128 inc, const ZigLds64* z,
                                              uint64_t& p) {
   ZigFeed<false> zf;
   double x;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBulkBlock) void norm_spec_kernel(const StreamDesc 
                                                                uint32_t* __restrict__ cnt,
                                                                uint64_t* __restrict__ mask,
                                                                uint64_t* __restrict__ exit) {
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   zig_load(&zig);
   __syncthreads();
   const uint64_t c = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBulkBlock) void norm_resolve_kernel(
     const uint64_t* __restrict__ spec_mask, const uint64_t* __restrict__ spec_exit,
     const uint64_t* __restrict__ prev_exit, uint64_t* __restrict__ cnt,
     uint64_t* __restrict__ texit, unsigned* __restrict__ changed) {
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   zig_load(&zig);
   __syncthreads();
   const uint64_t c = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kBulkBlock) void norm_emit_kernel(
     const uint64_t* __restrict__ cnt, const uint64_t* __restrict__ off, const uint64_t n,
     const double loc, const double scale, const int scaled, double* __restrict__ out,
     uint64_t* __restrict__ consumed) {
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   zig_load(&zig);
   __syncthreads();
   const uint64_t c = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(kBulkBlock) void seq_u64_kernel(const StreamDesc sd
 // One normal from the draws at q (q moves past it); false if it needs draws
 // at or past n.
 __device__ __forceinline__ bool normal_from(const uint64_t* __restrict__ U, uint64_t n,
-                                            uint64_t& q, const ZigLds* z, double& x) {
+                                            uint64_t& q, const ZigLds64* z, double& x) {
   ZigFeed<false> zf;
   for (;;) {
     if (q >= n) return false;
@@ -463,7 +463,7 @@ __device__ __forceinline__ bool normal_from(const uint64_t* __restrict__ U, uint
 // with kKeep its normals into zo and sqrt(nu / chi2) into *scale.
 template <bool kKeep>
 __device__ uint32_t t_sample(const uint64_t* __restrict__ U, uint64_t n, uint64_t p,
-                             const TParams& tp, const ZigLds* z, double* __restrict__ zo,
+                             const TParams& tp, const ZigLds64* z, double* __restrict__ zo,
                              double* __restrict__ scale) {
   uint64_t q = p;
   double x, v;
@@ -492,7 +492,7 @@ __device__ uint32_t t_sample(const uint64_t* __restrict__ U, uint64_t n, uint64_
 // different places of their samples).  phase 0: a gamma normal, 1: the gamma
 // uniform, 2 + j: the sample's normal j.
 __device__ uint32_t t_sample_len(const uint64_t* __restrict__ U, uint64_t n, uint64_t p,
-                                 const TParams& tp, const ZigLds* z) {
+                                 const TParams& tp, const ZigLds64* z) {
   ZigFeed<false> zf;
   const uint32_t end = 2u + tp.dim;
   uint32_t phase = 0;
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(kBulkBlock) void t_len_kernel(const uint64_t* __res
                                                            const uint64_t n_pos,
                                                            const TParams tp,
                                                            uint16_t* __restrict__ len) {
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   zig_load(&zig);
   __syncthreads();
   const uint64_t p = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(kBulkBlock) void t_emit_kernel(const uint64_t* __re
                                                             const TParams tp,
                                                             double* __restrict__ Z,
                                                             double* __restrict__ S) {
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   zig_load(&zig);
   __syncthreads();
   const uint64_t i = (uint64_t)blockIdx.x * kBulkBlock + threadIdx.x;

@@ -37,7 +37,7 @@ struct NormalSrc {
   using Rec = Real;
   static constexpr uint32_t kPerStep = kPer;
   static constexpr uint32_t kDefer = 8;  // slow-block deferral (queue_run)
-  static constexpr int kWaves = kFp32 ? 7 : 4;  // occupancy target of the fused kernel
+  static constexpr int kWaves = kFp32 ? 7 : 5;  // occupancy target of the fused kernel
   ZigWave<kFp32> zf;
   template <class C>
   __device__ __forceinline__ explicit NormalSrc(const C&) {}
@@ -74,8 +74,8 @@ struct MertonSrc {
   // and the uniform, waiting lanes cost more draws than the shared pass saves
   static constexpr uint32_t kDefer = 0;
   // FP32: 72 VGPRs, 23,040 B of LDS -- 7 waves per SIMD (1.10 -> 0.96 ms at
-  // 1M x 252 against the 4 the FP64 form runs)
-  static constexpr int kWaves = kFp32 ? 7 : 4;
+  // 1M x 252); FP64: 5
+  static constexpr int kWaves = kFp32 ? 7 : 5;
   ZigWave<kFp32> zf;
   uint64_t uni_next = 0;   // lanes whose next draw is the uniform
   uint64_t jump_next = 0;  // lanes whose next normal is the jump size
@@ -117,7 +117,7 @@ struct MertonSrc {
 template <class M>
 struct HestonQeSrc {
   using Real = double;
-  using Tables = ZigLds;
+  using Tables = ZigLds64;
   struct Rec {
     double vn, z1;
   };

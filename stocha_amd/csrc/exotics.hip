@@ -134,7 +134,7 @@ struct GbmObserved {
 template <class Src, class F, uint32_t kSlots>
 __device__ __forceinline__ GbmObserved<F> run_path(const GbmConst& c, const F* f,
                                                    const StreamDesc& sd, uint64_t p,
-                                                   uint32_t steps, const ZigLds* zig,
+                                                   uint32_t steps, const ZigLds64* zig,
                                                    typename Src::Rec (*q)[kFusedBlock]) {
   Pcg g{stream_at(&sd, p)};
   Src src(c);
@@ -150,7 +150,7 @@ exotic_sum_kernel(const GbmConst c, const F f, const StreamDesc sd, const uint64
                   const uint32_t steps, double* __restrict__ partials,
                   unsigned* __restrict__ counter, double* __restrict__ out) {
   using Src = GbmSrc<false>;
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   __shared__ Src::Rec q[8][kFusedBlock];
   __shared__ double red[kFusedBlock / 64][kMaxAcc];
   zig_load(&zig);
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(4))
 asian_pairs_kernel(const GbmConst c, const AsianFn f, const StreamDesc sd,
                    const uint64_t n_paths, const uint32_t steps, double2* __restrict__ out) {
   using Src = GbmSrc<false>;
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   __shared__ Src::Rec q[8][kFusedBlock];
   zig_load(&zig);
   __syncthreads();

@@ -130,7 +130,7 @@ template <bool kTiled>
 __global__ __launch_bounds__(kMultiBlock) void multi_feed_kernel(const MultiArgs a,
                                                                  const StreamDesc sd) {
   using Src = GbmSrc<false>;
-  __shared__ ZigLds zig;
+  __shared__ ZigLds64 zig;
   __shared__ double q[8][kMultiBlock];
   __shared__ double tile[kTiled ? kMultiTile : 1][kMultiBlock + 1];
   zig_load(&zig);

@@ -193,12 +193,16 @@ struct ZigLds32 {
   ZigFast32 fast32[512];
   float2 ydy32[256];  // FP32 wedge: (Y[l], Y[l+1] - Y[l])
 };
-// ... and what the FP64 samplers read on top (12 KB): the FP64 fast path, T
-// for the loop-form fast test and Y for the FP64 wedge.
-struct ZigLds : ZigLds32 {
+// What the FP64 queue samplers read (10 KB): the FP64 fast path and Y for the
+// FP64 wedge.
+struct ZigLds64 {
   ZigFast fast[512];
-  uint64_t t[256];  // T[l], exact fast-path threshold on m = u >> 11
   double y[257];
+};
+// Everything (18 KB), for kernels that also run the loop-form samplers, whose
+// fast test uses T.
+struct ZigLds : ZigLds32, ZigLds64 {
+  uint64_t t[256];  // T[l], exact fast-path threshold on m = u >> 11
 };
 
 __constant__ const double kZigX[257] = {ZIG_X_VALUES};
@@ -217,16 +221,19 @@ __device__ __forceinline__ void zig_load(ZigLds32* z) {
     if (i < 256) z->ydy32[i] = make_float2((float)kZigY[i], (float)(kZigY[i + 1] - kZigY[i]));
   }
 }
-__device__ __forceinline__ void zig_load(ZigLds* z) {
-  zig_load(static_cast<ZigLds32*>(z));
+__device__ __forceinline__ void zig_load(ZigLds64* z) {
   for (int i = threadIdx.x; i < 512; i += blockDim.x) {
     const int l = i & 255;
     z->fast[i].th = (uint32_t)(kZigT[l] >> 21);
     z->fast[i].pad = 0u;
     z->fast[i].x = (i & 256) ? -kZigX[l] : kZigX[l];
-    if (i < 256) z->t[i] = kZigT[i];
     if (i < 257) z->y[i] = kZigY[i];
   }
+}
+__device__ __forceinline__ void zig_load(ZigLds* z) {
+  zig_load(static_cast<ZigLds32*>(z));
+  zig_load(static_cast<ZigLds64*>(z));
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) z->t[i] = kZigT[i];
 }
 
 // next_f64 of a draw (pcg64.rs:37-39) rounded to FP32, from its bits: the

@@ -32,7 +32,7 @@ namespace stocha {
 template <bool kFp32>
 struct ZigFeed {
   using Real = typename std::conditional<kFp32, float, double>::type;
-  using Tables = typename std::conditional<kFp32, ZigLds32, ZigLds>::type;
+  using Tables = typename std::conditional<kFp32, ZigLds32, ZigLds64>::type;
   uint32_t state = 0;  // 0 attempt, 2 wedge uniform next, 1 / 3 tail pair
   uint32_t idx = 0;    // table index (layer | sign << 8) of a pending wedge / tail
   uint64_t ua = 0;     // wedge: the attempt draw; tail: the pair's first draw
@@ -143,7 +143,7 @@ __device__ __forceinline__ uint64_t lanes(bool c) { return __builtin_amdgcn_ball
 template <bool kFp32>
 struct ZigWave {
   using Real = typename std::conditional<kFp32, float, double>::type;
-  using Tables = typename std::conditional<kFp32, ZigLds32, ZigLds>::type;
+  using Tables = typename std::conditional<kFp32, ZigLds32, ZigLds64>::type;
   // The states after a missed attempt are told apart only when the lane is
   // served (in the slow block): the fast part neither tests the layer of a
   // miss nor keeps its sign.
