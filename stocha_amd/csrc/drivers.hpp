@@ -74,7 +74,7 @@ struct MertonSrc {
   // and the uniform, waiting lanes cost more draws than the shared pass saves
   static constexpr uint32_t kDefer = 0;
   // FP32: 72 VGPRs, 23,040 B of LDS -- 7 waves per SIMD (1.10 -> 0.96 ms at
-  // 1M x 252); FP64: 5
+  // 1M x 252); FP64: 91 VGPRs, 27,152 B -- 5 waves
   static constexpr int kWaves = kFp32 ? 7 : 5;
   ZigWave<kFp32> zf;
   uint64_t uni_next = 0;   // lanes whose next draw is the uniform

@@ -122,9 +122,10 @@ __global__ __launch_bounds__(kFusedBlock) void terminal_payoff_kernel(
 // the draw-count distribution of the ziggurat, a ring of 8 Heston normals
 // costs ~2.14 draws per step against the 2.06 minimum.
 // Occupancy target per source (Src::kWaves): the FP32 GBM/Heston/Merton
-// kernels run 7 waves per SIMD (<= 72 VGPRs, with three dwords spilled
+// kernels run 7 waves per SIMD (<= 72 VGPRs, with two dwords spilled
 // outside the path loop; 23,040 bytes of LDS per block, which seven blocks
-// fit in 160 KB); the FP64 ones need more registers and run 4.
+// fit in 160 KB); the FP64 ones run 5 (87-91 VGPRs, no spills; 27,152 bytes
+// of LDS, the 10 KB FP64 table block and a 16 KB ring).
 template <class Src, class Model, class Const, class Payoff, int kQueue>
 __global__ __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(
     Src::kWaves))) void terminal_feed_kernel(

@@ -188,7 +188,7 @@ struct ZigFast32 {
 // FP32 wedge.  The exact threshold T and the FP64 X and Y, read only on a tied
 // fast test or inside the FP32 wedge's guard band (a few draws in a million),
 // come from constant memory (kZigT, kZigX, kZigY), which keeps the block's LDS
-// small enough for six resident blocks per CU.
+// small enough for seven resident blocks per CU.
 struct ZigLds32 {
   ZigFast32 fast32[512];
   float2 ydy32[256];  // FP32 wedge: (Y[l], Y[l+1] - Y[l])
