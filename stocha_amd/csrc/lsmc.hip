@@ -73,26 +73,34 @@ struct DateMajorHook {
   }
 };
 
+// Block size of the path kernel: FP32 blocks of 512 threads (38,912 B of
+// LDS, 48 VGPRs) run 8 waves per SIMD, so 1,048,576 paths take exactly two
+// rounds of resident threads (256-thread blocks: 22,528 B, 7 waves per SIMD,
+// 2.3 rounds -- three path lengths); FP64 fits 6 waves either way.
 template <bool kFp32>
-__global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, const StreamDesc sd,
-                                                            const uint64_t n, const uint32_t steps,
-                                                            void* __restrict__ paths,
-                                                            double* __restrict__ cf, const double k,
-                                                            const int is_put) {
+constexpr int gen_block() {
+  return kFp32 ? 512 : kLsBlock;
+}
+
+template <bool kFp32>
+__global__ __launch_bounds__(gen_block<kFp32>()) void lsmc_gen_kernel(
+    const GbmConst mc, const StreamDesc sd, const uint64_t n, const uint32_t steps,
+    void* __restrict__ paths, double* __restrict__ cf, const double k, const int is_put) {
   using Src = GbmSrc<kFp32>;
   using Model = typename std::conditional<kFp32, Gbm32, Gbm64>::type;
   constexpr uint32_t kSlots = kFp32 ? 16 : 8;
+  constexpr int kB = gen_block<kFp32>();
   __shared__ typename Src::Tables zig;
-  __shared__ typename Src::Rec q[kSlots][kLsBlock];
+  __shared__ typename Src::Rec q[kSlots][kB];
   zig_load(&zig);
   __syncthreads();
-  const uint64_t stride = (uint64_t)gridDim.x * kLsBlock;
+  const uint64_t stride = (uint64_t)gridDim.x * kB;
   // the first state of this thread's next path: stream_at once, then one
   // affine step of `stride` streams per path (50-step paths would otherwise
   // spend ~15% of their work finding their streams)
   const Jump next = stream_stride(&sd, stride);
-  U128 first = stream_at(&sd, (uint64_t)blockIdx.x * kLsBlock + threadIdx.x);
-  for (uint64_t base = (uint64_t)blockIdx.x * kLsBlock; base < n; base += stride) {
+  U128 first = stream_at(&sd, (uint64_t)blockIdx.x * kB + threadIdx.x);
+  for (uint64_t base = (uint64_t)blockIdx.x * kB; base < n; base += stride) {
     const uint64_t i = base + threadIdx.x;
     DateMajorHook<kFp32> hook;
     hook.paths = paths;
@@ -102,7 +110,7 @@ __global__ __launch_bounds__(kLsBlock) void lsmc_gen_kernel(const GbmConst mc, c
     hook.s0f = (float)mc.s0;
     Pcg g{first};
     first = add128(mul128(next.a, first), next.c);
-    (void)queue_path<Src, Model, kSlots, kLsBlock>(mc, g, sd.inc, steps, &zig, q, false, hook);
+    (void)queue_path<Src, Model, kSlots, kB>(mc, g, sd.inc, steps, &zig, q, false, hook);
     if (cf != nullptr && hook.valid) {
       double last;
       if constexpr (kFp32)
@@ -604,32 +612,68 @@ __device__ __forceinline__ double coop_block_sum(const double* acc, int na,
   return s;
 }
 
-// Every block: fixed-order total over blocks of part[e * nb + b] (element e
-// of block b's partial, e < na <= 16) -> tot (LDS).  Wave e sums element e:
-// lane l adds blocks l, l + 64, ... (coalesced loads), then the lanes in a
-// fixed butterfly; one block barrier.
-__device__ __forceinline__ void coop_total(const double* part, int na, double* tot) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned nb = gridDim.x;
-  if (wave < na) {
-    const double* col = part + (uint64_t)wave * nb;
-    double s = 0.0;
-    for (unsigned b0 = 0; b0 < nb; b0 += 256) {  // 4 loads in flight per lane
-      double v[4];
-      uint64_t bits[4];
+// The fixed-order total over blocks of column `col` (nb slots, one per
+// block) for the calling wave: lane l adds blocks l, l + 64, ... (coalesced
+// loads), then the lanes in a fixed butterfly.  The wave loads its slots
+// together and re-loads all the still-empty ones together, so waiting for
+// late blocks costs one load round trip per poll, not one per empty slot.
+// The poll is bounded (2^24 rounds, ~0.5 s of sleeps): a slot that never
+// fills yields a NaN total, never a hang.
+__device__ __forceinline__ double column_total(const double* col, unsigned nb) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (unsigned b0 = 0; b0 < nb; b0 += 256) {  // 4 loads in flight per lane
+    uint64_t bits[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {  // unpredicated loads (block 0 past nb)
-        const unsigned b = b0 + 64 * k + lane;
-        bits[k] = load_slot(&col[b < nb ? b : 0u]);
+    for (int k = 0; k < 4; ++k) {  // unpredicated loads (block 0 past nb)
+      const unsigned b = b0 + 64 * k + lane;
+      bits[k] = load_slot(&col[b < nb ? b : 0u]);
+    }
+    for (uint32_t round = 0;; ++round) {
+      bool empty = false;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) empty |= (b0 + 64 * k + lane < nb) & (bits[k] == kEmptySlot);
+      if (!__builtin_amdgcn_ballot_w64(empty)) break;
+      if (round == (1u << 24)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          bits[k] = bits[k] == kEmptySlot ? 0x7ff8000000000000ull : bits[k];  // NaN
+        break;
       }
+      __builtin_amdgcn_s_sleep(1);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const unsigned b = b0 + 64 * k + lane;
-        v[k] = b < nb ? slot_value(const_cast<double*>(&col[b]), bits[k]) : 0.0;
+        if (bits[k] == kEmptySlot) bits[k] = load_slot(&col[b < nb ? b : 0u]);
       }
-      s += (v[0] + v[1]) + (v[2] + v[3]);
     }
-    s = wave_sum_d(s);
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      v[k] = b0 + 64 * k + lane < nb ? __longlong_as_double((long long)bits[k]) : 0.0;
+    s += (v[0] + v[1]) + (v[2] + v[3]);
+  }
+  return wave_sum_d(s);
+}
+
+// Every block: fixed-order totals over blocks of part[e * nb + b] (element e
+// of block b's partial, e < na <= 16) -> tot (LDS); wave e sums element e.
+// A block's na elements are stored by one store instruction, so once every
+// block's element 0 is there the others are too, bar stragglers (which their
+// waves' own polls wait for).  So wave 0 alone polls column 0 while the
+// other waves wait at a block barrier, and then read their columns once:
+// with all na waves polling together the polls' traffic slows the stores
+// they wait for (measured on 256 blocks: 4.3 us per exchange, 2.4 us this way).
+__device__ __forceinline__ void coop_total(const double* part, int na, double* tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned nb = gridDim.x;
+  if (wave == 0) {
+    const double s = column_total(part, nb);
+    if (lane == 0) tot[0] = s;
+  }
+  lds_barrier();
+  if (wave >= 1 && wave < na) {
+    const double s = column_total(part + (uint64_t)wave * nb, nb);
     if (lane == 0) tot[wave] = s;
   }
   lds_barrier();
@@ -783,8 +827,11 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     }
     double* pp = a.part + (uint64_t)date * pstride;
     const double bs = coop_block_sum<kCoopWaves>(acc, na, red);
+    // The storing wave waits for its store before it polls (its loads
+    // would otherwise hold the store up: measured 2.9 -> 1.3 us per
+    // one-word exchange); the other waves wait in coop_total anyway.
     if (threadIdx.x < na) st_agent(&pp[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x], bs);
-    __syncthreads();
+    if (threadIdx.x < 64) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     coop_total(pp, na, tot);
     // The spots of date-1 are loaded here: after the totals' loads (vector
     // memory returns in order), and ahead of the solve and the next exercise
@@ -947,8 +994,13 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
   // itself (no cf array) and starts from empty partial slots
   if (coop)
     if (int rc = fill_empty_slots(part, part_n, st, err)) return rc;
-  lsmc_gen_kernel<kFp32><<<nb, kLsBlock, 0, st>>>(gc, sd, n, (uint32_t)steps, paths,
-                                                  coop ? nullptr : cf, k, is_put);
+  {
+    constexpr int kGb = gen_block<kFp32>();
+    const uint64_t gb64 = (n + kGb - 1) / kGb;
+    const int gnb = (int)(gb64 > 2048 ? 2048 : gb64);
+    lsmc_gen_kernel<kFp32><<<gnb, kGb, 0, st>>>(gc, sd, n, (uint32_t)steps, paths,
+                                                coop ? nullptr : cf, k, is_put);
+  }
   STOCHA_HIP(hipGetLastError());
   if (coop) {
     LsmcCoopArgs ca;
