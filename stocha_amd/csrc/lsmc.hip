@@ -455,10 +455,14 @@ struct LsmcCoopArgs {
 // q < ncols): sum x^e = sum_q C(e, q) al^q be^(e-q) tot[q], and the same for
 // sum cf x^e with tot[yoff + q].  Lane e < nm of the calling wave writes g[e],
 // lane 16 + e (e < ncols) writes h[e].  Per lane the sum is a Horner scheme
-// in be over q = 0..e (a few instructions per q on the statistics wave's
-// serial path).
+// in be over q = 0..e, one fused multiply-add per q on the statistics wave's
+// serial path; C(e, q) = rint(e (e-1) ... (e-q+1) / q!) comes off a
+// falling-factorial product (exact integers below 2^16; the rounded quotient
+// lies within 1e-12 of the integer), which is not on that path.
 __device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff, double al,
                                           double be, double* g, double* h) {
+  constexpr double kInvFact[kMaxMom] = {1.0,       1.0,        1.0 / 2,     1.0 / 6,     1.0 / 24,
+                                        1.0 / 120, 1.0 / 720,  1.0 / 5040,  1.0 / 40320};
   const int nm = 2 * (ncols - 1) + 1;
   int lane = threadIdx.x & 63;
   asm volatile("" : "+v"(lane));  // lane-dependent values are formed here, not kept live
@@ -466,14 +470,16 @@ __device__ __forceinline__ void moments_x(const double* tot, int ncols, int yoff
   if (!is_g && !is_h) return;
   const int e = is_g ? lane : lane - 16;
   const double* src = is_g ? tot : tot + yoff;
-  double s = 0.0, binom = 1.0, aq = 1.0;  // binom = C(e, q), aq = al^q
+  double s = 0.0, ff = 1.0, aq = 1.0;  // ff = e (e-1) ... (e-q+1), aq = al^q
 #pragma unroll
   for (int q = 0; q < kMaxMom; ++q) {
-    const double term = binom * aq * src[q <= e ? q : 0];  // reads stay below nm / ncols
-    s = q <= e ? s * be + term : s;
-    // C(e, q+1) = C(e, q) (e - q) / (q + 1): the product is an exact integer
-    // below 2^10 and the rounded quotient lies within 1e-12 of the integer.
-    binom = rint(binom * (double)(e - q) * (1.0 / (q + 1)));
+    // every lane reads (index min(q, e) stays below nm / ncols): a read
+    // under the q <= e condition would be a branch and an LDS wait per q
+    const double v = src[q < e ? q : e];
+    const double binom = rint(ff * kInvFact[q]);  // C(e, q)
+    const double term = binom * aq * v;
+    s = q <= e ? __builtin_fma(s, be, term) : s;
+    ff *= (double)(e - q);
     aq *= al;
   }
   if (is_g)
@@ -493,6 +499,16 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
   return __hiloint2double(hi, lo);
+}
+
+// 1 / d from the hardware reciprocal and two Newton steps (within an ulp of
+// the quotient; the solve's own rounding is coarser than that).
+__device__ __forceinline__ double recip(double d) {
+  double x = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, x, 1.0);
+  x = __builtin_fma(x, e, x);
+  e = __builtin_fma(-d, x, 1.0);
+  return __builtin_fma(x, e, x);
 }
 
 // G c = h with G[i][j] = g[i + j] (i, j < ncols) in FP64: the least-squares
@@ -517,12 +533,14 @@ __device__ __forceinline__ void solve_normal_eq_wave(const double* g, const doub
   double my_inv = 1.0;  // lane i: 1 / (pivot i)
 #pragma unroll
   for (int k = 0; k < kMaxCols; ++k) {
+    // pivots past ncols are the identity rows': they leave rows < ncols alone
+    if (k >= ncols) break;
     double pk[kMaxCols];
 #pragma unroll
     for (int j = k; j < kMaxCols; ++j) pk[j] = readlane_d(r[j], k);
     const double phk = readlane_d(hh, k);
-    if (k < ncols) ok = ok && pk[k] > 0.0;
-    const double inv = 1.0 / pk[k];
+    ok = ok && pk[k] > 0.0;
+    const double inv = recip(pk[k]);
     my_inv = lane == k ? inv : my_inv;
     const double f = lane != k ? r[k] * inv : 0.0;
 #pragma unroll
@@ -712,7 +730,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   __shared__ double tot[kMaxAccB];
   __shared__ double coef_s[kMaxCols];
   __shared__ double g_s[kMaxMom], h_s[kMaxCols];
-  __shared__ double ms_s[2];
+  __shared__ double ms_s[3];
   __shared__ int skip_s;
   // n <= kCoopMaxPpt x the grid's threads (< 2^32): path indices are 32-bit,
   // and a spot's address is a per-date column base plus a 32-bit offset.
@@ -760,6 +778,7 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
   double ex_mean = 0.0, ex_std = 1.0, ex_inv = 1.0, ex_c[kMaxCols];
   for (int j = 0; j < kMaxCols; ++j) ex_c[j] = 0.0;
   double c = a.k, sc = a.k;  // moment variable u = (S - c) / sc
+  double inv_sc = 1.0 / a.k;
   // Up to 4 paths per thread, with kRegSpot, and at 16 FP64 paths per thread
   // (registers to spare there) the spots of the next date are loaded a pass
   // ahead, so their latency overlaps the solve and the exercise pass.
@@ -777,7 +796,6 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     double acc[kMaxAccB];
 #pragma unroll
     for (int e = 0; e < kMaxAccB; ++e) acc[e] = 0.0;
-    const double inv_sc = uniform_d(1.0 / sc);
     const SpotCol<kFp32> col0(a.paths, a.n, date), col1(a.paths, a.n, date + 1);
     const uint32_t toff = fresh_offset(t0, sizeof(SpotT));
     // exercise at date+1 with its regression, then one discount step
@@ -848,16 +866,19 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
       const double m = tot[0];
       const int skip = (uint64_t)m < (uint64_t)ncols + 1;  // lsmc.rs:62-64
       if (!skip) {
-        const double mu = tot[1] / m;
-        double var = tot[2] / m - mu * mu;
+        const double inv_m = recip(m);
+        const double mu = tot[1] * inv_m;
+        double var = tot[2] * inv_m - mu * mu;
         var = var > 0.0 ? var : 0.0;
         const double mean = c + sc * mu;
         double sdv = sqrt(var) * sc;
         sdv = sdv > 1e-12 ? sdv : 1e-12;  // lsmc.rs:67-71
-        moments_x(tot, ncols, kMaxMom, sc / sdv, (c - mean) / sdv, g_s, h_s);
+        const double inv_sdv = recip(sdv);
+        moments_x(tot, ncols, kMaxMom, sc * inv_sdv, (c - mean) * inv_sdv, g_s, h_s);
         if (threadIdx.x == 0) {
           ms_s[0] = mean;
           ms_s[1] = sdv;
+          ms_s[2] = inv_sdv;
         }
       }
       if (threadIdx.x == 0) skip_s = skip;
@@ -875,11 +896,12 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
     if (have_ex) {  // block-uniform values, held in scalar registers
       ex_mean = uniform_d(ms_s[0]);
       ex_std = uniform_d(ms_s[1]);
-      ex_inv = uniform_d(1.0 / ex_std);
+      ex_inv = uniform_d(ms_s[2]);
 #pragma unroll
       for (int j = 0; j < kMaxCols; ++j) ex_c[j] = uniform_d(coef_s[j]);
       c = ex_mean;
       sc = ex_std;
+      inv_sc = ex_inv;
     }
     if (date == 1) break;
   }
