@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kSumsBlock) __attribute__((amdgpu_waves_per_eu(
   const bool valid = p < np;
   Src src(bd);
   Pcg g{stream_at(&sd, p0 + (valid ? p : 0))};  // lanes past the end replay a path, unstored
-  const U128 incv = vgpr_increment(sd.inc);
+  const LcgInc incv = vgpr_increment(sd.inc);
   const uint32_t d = bd.d;
   const uint32_t nc = (d + kChunk - 1) / kChunk;  // <= NC
   Real W[kChunk * NC];

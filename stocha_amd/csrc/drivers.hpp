@@ -205,17 +205,18 @@ struct NoHook {
   __device__ __forceinline__ void operator()(uint32_t, const M&) const {}
 };
 
-// The stream increment with the two words that meet a carry-in (v_addc) in
-// VGPRs for the whole loop: an SGPR operand next to VCC would exceed gfx9's
-// one scalar operand per VALU instruction and cost a v_mov per draw.
-__device__ __forceinline__ U128 vgpr_increment(const U128 inc) {
-  U128 incv = inc;
-  uint32_t w1 = (uint32_t)(inc.lo >> 32), w2 = (uint32_t)inc.hi;
-  asm("" : "+v"(w1));
-  asm("" : "+v"(w2));
-  incv.lo = ((uint64_t)w1 << 32) | (uint32_t)inc.lo;
-  incv.hi = (inc.hi & 0xffffffff00000000ull) | w2;
-  return incv;
+// The stream increment as two 64-bit halves in VGPR pairs for the whole
+// loop: they are addends of Pcg::draw_step's multiply-adds, whose other
+// scalar operand is the multiplier (gfx9 allows one scalar operand per VALU
+// instruction).
+struct LcgInc {
+  uint64_t lo, hi;
+};
+__device__ __forceinline__ LcgInc vgpr_increment(const U128 inc) {
+  LcgInc r{inc.lo, inc.hi};
+  asm("" : "+v"(r.lo));
+  asm("" : "+v"(r.hi));
+  return r;
 }
 
 // c + 1 on the lanes of mask m, c elsewhere: one add with the lane mask as
@@ -234,7 +235,7 @@ __device__ __forceinline__ uint32_t add_lanes(uint32_t c, uint64_t m) {
 // Waiting lanes (their next draw goes to the slow block) are served once
 // Src::kDefer of them wait or one of them lacks entries.
 template <class Src, uint32_t kSlots, int kCols>
-__device__ __forceinline__ void queue_fill(Src& src, Pcg& g, const U128& incv,
+__device__ __forceinline__ void queue_fill(Src& src, Pcg& g, const LcgInc& incv,
                                            const typename Src::Tables* zig,
                                            typename Src::Rec (*q)[kCols], uint32_t head,
                                            uint32_t& cnt, uint32_t need_n) {
@@ -258,11 +259,7 @@ __device__ __forceinline__ void queue_fill(Src& src, Pcg& g, const U128& incv,
       const bool go = (n_wait >= Src::kDefer) | ((waiting & need) != 0);
       act &= go ? ~0ull : ~waiting;
     }
-    const uint64_t u = g.output();
-    const U128 sn = g.stepped(incv);
-    const bool a = lane_in(act);
-    g.s.lo = a ? sn.lo : g.s.lo;
-    g.s.hi = a ? sn.hi : g.s.hi;
+    const uint64_t u = g.draw_step(incv.lo, incv.hi, act);  // steps the act lanes
     Rec r;
     const uint64_t got = src.feed(u, zig, r, act);
     q[(head + cnt) & (kSlots - 1)][threadIdx.x] = r;
@@ -279,7 +276,7 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
                                           typename Src::Rec (*q)[kCols], Hook hook = Hook()) {
   static_assert((kSlots & (kSlots - 1)) == 0 && kSlots > Src::kPerStep, "ring size");
   constexpr uint32_t kNeed = Src::kPerStep;
-  const U128 incv = vgpr_increment(inc);
+  const LcgInc incv = vgpr_increment(inc);
   uint32_t head = 0, cnt = 0;
   for (uint32_t step = 0; step < steps; ++step) {
     queue_fill<Src, kSlots, kCols>(src, g, incv, zig, q, head, cnt, kNeed);

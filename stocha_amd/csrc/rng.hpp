@@ -105,6 +105,80 @@ struct Pcg {
     return o;
   }
 
+  // The DXSM output of the current state, and the LCG step applied in place
+  // on the lanes of `act` only (the others keep their state), for the queue
+  // schedule's fill loop.  The output is written on 32-bit limbs so that
+  // h1 * m0, a term of both the output's first multiply and the step's, is
+  // formed once.  The step s * M + inc (mod 2^128) is
+  //   A = l0 m0 + inc_lo  (carry cA at 2^64)      D = l0 m1 + l1 m0  (carry cD at 2^96)
+  //   Q = l1 m1 + h0 m0 + inc_hi (mod 2^64)       x = h0 m1 + h1 m0  (mod 2^32)
+  //   word 0 = A.lo, word 1 = A.hi + D.lo (carry e1), word 2 = Q.lo + D.hi + cA + e1,
+  //   word 3 = Q.hi + x + cD + the carries of word 2,
+  // in gfx950 code: the multiply-adds take the increment's 64-bit halves and
+  // each other's whole results as addends and hand their carries out in
+  // scalar registers (no partial product is moved into a register pair), and
+  // the four result words are written into the state under an execution mask
+  // of the `act` lanes (so no selects commit the new state).  A VALU write of
+  // a scalar register needs two wait states before a VALU reads it as a
+  // carry; the instruction order provides them.  inc_lo / inc_hi: the
+  // increment's halves in VGPR pairs (the multiply-adds already read the
+  // multiplier from a scalar register).  The execution mask is restored
+  // before the block ends.
+  __device__ __forceinline__ uint64_t draw_step(const uint64_t inc_lo, const uint64_t inc_hi,
+                                                const uint64_t act) {
+    constexpr uint32_t m0 = (uint32_t)kPcgMul, m1 = (uint32_t)(kPcgMul >> 32);
+    uint32_t l0 = (uint32_t)s.lo, l1 = (uint32_t)(s.lo >> 32);
+    uint32_t h0 = (uint32_t)s.hi, h1 = (uint32_t)(s.hi >> 32);
+    // DXSM (rand_pcg output_dxsm): hi ^= hi >> 32; hi *= M; hi ^= hi >> 48; hi *= lo | 1
+    const uint32_t h1m0 = h1 * m0;
+    const uint32_t x0 = h0 ^ h1;
+    const uint64_t p = (uint64_t)x0 * m0;
+    const uint32_t y1 = (uint32_t)(p >> 32) + x0 * m1 + h1m0;
+    const uint32_t y0 = (uint32_t)p ^ (y1 >> 16);
+    const uint32_t o0 = l0 | 1u;
+    const uint64_t q = (uint64_t)y0 * o0;
+    const uint64_t out = ((uint64_t)((uint32_t)(q >> 32) + y0 * l1 + y1 * o0) << 32) | (uint32_t)q;
+    uint64_t A, D, Q, cA, cD, dm;
+    uint32_t cr;
+    asm volatile(
+        "v_mad_u64_u32 %[A], %[cA], %[l0], %[m0], %[il]\n\t"
+        "v_mad_u64_u32 %[D], %[dm], %[l0], %[m1], 0\n\t"
+        "v_mad_u64_u32 %[Q], %[dm], %[h0], %[m0], %[ih]\n\t"
+        "v_mul_lo_u32 %[cr], %[h0], %[m1]\n\t"
+        "v_mad_u64_u32 %[D], %[cD], %[l1], %[m0], %[D]\n\t"
+        "v_mad_u64_u32 %[Q], %[dm], %[l1], %[m1], %[Q]"
+        : [A] "=&v"(A), [D] "=&v"(D), [Q] "=&v"(Q), [cr] "=&v"(cr), [cA] "=&s"(cA),
+          [cD] "=&s"(cD), [dm] "=&s"(dm)
+        : [l0] "v"(l0), [l1] "v"(l1), [h0] "v"(h0), [m0] "s"(m0), [m1] "s"(m1),
+          [il] "v"(inc_lo), [ih] "v"(inc_hi));
+    // the four words into the state on the act lanes (cA and cD were written
+    // at least four instructions before their first read)
+    uint64_t e1, f1, f2, dm2, save;
+    uint32_t t;
+    asm volatile(
+        "s_and_saveexec_b64 %[save], %[act]\n\t"
+        "v_mov_b32 %[l0], %[alo]\n\t"
+        "v_add_co_u32_e64 %[l1], %[e1], %[ahi], %[dlo]\n\t"
+        "v_addc_co_u32_e64 %[h0], %[f1], %[qlo], %[dhi], %[cA]\n\t"
+        "v_add3_u32 %[t], %[qhi], %[cr], %[h1m0]\n\t"
+        "v_addc_co_u32_e64 %[h0], %[f2], %[h0], 0, %[e1]\n\t"
+        "v_addc_co_u32_e64 %[t], %[dm], %[t], 0, %[cD]\n\t"
+        "v_addc_co_u32_e64 %[t], %[dm], %[t], 0, %[f1]\n\t"
+        "v_addc_co_u32_e64 %[h1], %[dm], %[t], 0, %[f2]\n\t"
+        "s_mov_b64 exec, %[save]"
+        : [l0] "+v"(l0), [l1] "+v"(l1), [h0] "+v"(h0), [h1] "+v"(h1), [t] "=&v"(t),
+          [e1] "=&s"(e1), [f1] "=&s"(f1), [f2] "=&s"(f2), [dm] "=&s"(dm2), [save] "=&s"(save)
+        : [alo] "v"((uint32_t)A), [ahi] "v"((uint32_t)(A >> 32)), [dlo] "v"((uint32_t)D),
+          [dhi] "v"((uint32_t)(D >> 32)), [qlo] "v"((uint32_t)Q),
+          [qhi] "v"((uint32_t)(Q >> 32)), [cr] "v"(cr), [h1m0] "v"(h1m0), [cA] "s"(cA),
+          [cD] "s"(cD), [act] "s"(act));
+    (void)dm;
+    (void)dm2;
+    s.lo = ((uint64_t)l1 << 32) | l0;
+    s.hi = ((uint64_t)h1 << 32) | h0;
+    return out;
+  }
+
   // pcg64.rs:37-39: (u >> 11) * 2^-53, exact
   __device__ __forceinline__ double next_f64(const U128 inc) {
     return (double)(next(inc) >> 11) * 0x1p-53;
