@@ -123,7 +123,7 @@ struct Pcg {
   // carry; the instruction order provides them.  inc_lo / inc_hi: the
   // increment's halves in VGPR pairs (the multiply-adds already read the
   // multiplier from a scalar register).  The execution mask is restored
-  // before the block ends.
+  // before the block ends; the block sets SCC (s_and_saveexec), and says so.
   __device__ __forceinline__ uint64_t draw_step(const uint64_t inc_lo, const uint64_t inc_hi,
                                                 const uint64_t act) {
     constexpr uint32_t m0 = (uint32_t)kPcgMul, m1 = (uint32_t)(kPcgMul >> 32);
@@ -171,7 +171,8 @@ struct Pcg {
         : [alo] "v"((uint32_t)A), [ahi] "v"((uint32_t)(A >> 32)), [dlo] "v"((uint32_t)D),
           [dhi] "v"((uint32_t)(D >> 32)), [qlo] "v"((uint32_t)Q),
           [qhi] "v"((uint32_t)(Q >> 32)), [cr] "v"(cr), [h1m0] "v"(h1m0), [cA] "s"(cA),
-          [cD] "s"(cD), [act] "s"(act));
+          [cD] "s"(cD), [act] "s"(act)
+        : "scc");  // s_and_saveexec sets SCC
     (void)dm;
     (void)dm2;
     s.lo = ((uint64_t)l1 << 32) | l0;

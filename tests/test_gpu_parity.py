@@ -177,6 +177,18 @@ def test_fused_fp64_matches_oracle(gpu, orc, model, params):
     np.testing.assert_allclose(res["sumsq"], s2, rtol=RTOL64)
 
 
+@pytest.mark.parametrize("model,params", [("gbm", GBM), ("heston", HESTON)])
+def test_fused_fp64_paths_past_a_million_steps(gpu, orc, model, params):
+    """One wavefront of paths 2^20 + 3 steps long through the queue schedule
+    (its ring head wraps ~2^17 times): same draws, same sums as the oracle."""
+    n, steps = 64, (1 << 20) + 3
+    res = gpu.european_price(model, params, 1.0, steps, n, STRIKES, r=0.05, seed=7,
+                             precision=64)
+    ref, s1, s2 = _oracle_prices(orc, model, params, steps, n, 7)
+    np.testing.assert_allclose(res["sum"], s1, rtol=RTOL64)
+    np.testing.assert_allclose(res["sumsq"], s2, rtol=RTOL64)
+
+
 @pytest.mark.parametrize("model,params", [("gbm", GBM), ("heston", HESTON),
                                           ("merton", MERTON)])
 def test_fused_fp32_within_1e4(gpu, orc, model, params):
