@@ -12,6 +12,9 @@
 // and diffusion are uniform and read with scalar loads.  Up to kMultiTile
 // values per row are staged in an LDS tile and written row segment by row
 // segment (d <= kMultiTile; larger d writes each row's d values directly).
+// For d <= kRegD (multi_reg_kernel) a step's normals and the row's prices sit
+// in registers instead: the queue fill asks each lane for d normals at once,
+// and the step reads them from the ring in asset order.
 // With corr = [[1]] this is bit-for-bit the GBM stepper (same block size, L=1).
 #include <math.h>
 
@@ -60,6 +63,34 @@ struct MultiArgs {
   uint32_t d, steps;
 };
 
+// Writes the tile's tpos values of each of the block's rows (output columns
+// col0 .. col0+tpos-1) and empties the tile.  Every thread of the block calls
+// it together.  Thread t owns tile column t % kMultiTile and rows t /
+// kMultiTile + kFlushRows * j: each pass stores kFlushRows row segments of up
+// to kMultiTile consecutive values (16 lanes per 128-byte segment), one
+// pointer add per store.  (LDS-only barriers: the row stores stay in flight
+// across them.)
+constexpr uint32_t kFlushRows = kMultiBlock / kMultiTile;
+static_assert(kMultiBlock % kMultiTile == 0, "flush layout");
+__device__ __forceinline__ void flush_tile(const MultiArgs& a, double (*tile)[kMultiBlock + 1],
+                                           uint32_t& tpos, uint64_t& col0) {
+  lds_barrier();
+  const uint64_t row0 = (uint64_t)blockIdx.x * kMultiBlock;
+  const uint64_t left = a.n_rows - row0;
+  const uint32_t rows_here = left < (uint64_t)kMultiBlock ? (uint32_t)left : kMultiBlock;
+  const uint64_t width = (a.steps + 1ull) * a.d;
+  const uint32_t c = threadIdx.x % kMultiTile;
+  const uint32_t r0 = threadIdx.x / kMultiTile;
+  if (c < tpos) {
+    double* p = a.out + (row0 + r0) * width + col0 + c;
+    const uint64_t pass = (uint64_t)kFlushRows * width;
+    for (uint32_t r = r0; r < rows_here; r += kFlushRows, p += pass) *p = tile[c][r];
+  }
+  lds_barrier();
+  col0 += tpos;
+  tpos = 0;
+}
+
 // The queue model: one normal per apply; the d-th completes a step
 // (multi_gbm.rs:60-76).  Every lane of the block completes its steps on the
 // same queue steps, so the tile flushes' barriers line up.
@@ -85,23 +116,7 @@ struct MultiModel {
     // invariant before every step: the step's d values fit (tpos + d <= tcap)
     if (kTiled && tpos + a.d > tcap) flush();
   }
-  // (LDS-only barriers: the row stores stay in flight across them)
-  __device__ __forceinline__ void flush() {
-    lds_barrier();
-    const uint64_t row0 = (uint64_t)blockIdx.x * kMultiBlock;
-    const uint64_t left = a.n_rows - row0;
-    const int rows_here = left < (uint64_t)kMultiBlock ? (int)left : kMultiBlock;
-    const int n_el = rows_here * (int)tpos;
-    const uint64_t width = (a.steps + 1ull) * a.d;
-    for (int e = threadIdx.x; e < n_el; e += kMultiBlock) {
-      const int r = e / (int)tpos;
-      const int c = e - r * (int)tpos;
-      a.out[(row0 + r) * width + col0 + c] = tile[c][r];
-    }
-    lds_barrier();
-    col0 += tpos;
-    tpos = 0;
-  }
+  __device__ __forceinline__ void flush() { flush_tile(a, tile, tpos, col0); }
   __device__ __forceinline__ void apply(double z) {
     a.zs[(uint64_t)k * a.stride + row] = mirror ? -z : z;  // sample_into (normal.rs:70-74)
     if (++k < a.d) return;
@@ -146,6 +161,78 @@ __global__ __launch_bounds__(kMultiBlock) void multi_feed_kernel(const MultiArgs
   queue_run<Src, 8, kMultiBlock>(src, m, g, sd.inc, a.steps * a.d, &zig, q);
 }
 
+// d <= kRegD: one kernel per d (template D), with the step's normals z and
+// the row's prices s in registers, and L's lower triangle, drift and
+// diffusion too: held in VGPRs for the whole loop (as scalars they would not
+// fit the SGPR file at d = 8 and spill).  With d fixed at compile time the d
+// exp chains of a step are independent straight-line code the compiler
+// interleaves.  A queue fill leaves every lane at least D normals in its ring
+// (kRegSlots - 1 entries at most); the step takes them in draw order, so each
+// row consumes its stream exactly as the one-normal-per-step form does.  The
+// arithmetic is multi_gbm.rs:67-74's, term for term: eps from 0.0 by products
+// and sums in k order, then s * exp(drift + diffusion * eps).
+constexpr uint32_t kRegD = 8;
+constexpr uint32_t kRegSlots = 16;
+
+template <uint32_t D>
+__global__ __launch_bounds__(kMultiBlock) void multi_reg_kernel(const MultiArgs a,
+                                                                const StreamDesc sd) {
+  using Src = GbmSrc<false>;
+  static_assert(D >= 1 && D <= kRegD && kRegSlots > kRegD && kMultiTile >= kRegD,
+                "ring and tile sizes");
+  constexpr uint32_t kCap = (kMultiTile / D) * D;  // whole steps per tile (>= 2 steps)
+  __shared__ ZigLds64 zig;
+  __shared__ double q[kRegSlots][kMultiBlock];
+  __shared__ double tile[kMultiTile][kMultiBlock + 1];
+  zig_load(&zig);
+  const uint64_t row = (uint64_t)blockIdx.x * kMultiBlock + threadIdx.x;
+  const bool valid = row < a.n_rows;
+  const bool mirror = valid && row >= a.base_rows;
+  const uint64_t path = !valid ? 0 : (mirror ? row - a.base_rows : row);
+  Pcg g{stream_at(&sd, path)};
+  double s[D], lt[D * (D + 1) / 2], dr[D], df[D];
+#pragma unroll
+  for (uint32_t i = 0; i < D; ++i) {
+#pragma unroll
+    for (uint32_t k = 0; k <= i; ++k) {
+      lt[i * (i + 1) / 2 + k] = a.L[i * D + k];
+      asm("" : "+v"(lt[i * (i + 1) / 2 + k]));
+    }
+    dr[i] = a.drift[i];
+    df[i] = a.diff[i];
+    asm("" : "+v"(dr[i]), "+v"(df[i]));
+    s[i] = a.s0[i];
+    tile[i][threadIdx.x] = s[i];
+  }
+  uint32_t tpos = D;
+  uint64_t col0 = 0;
+  __syncthreads();
+  Src src(a);
+  const LcgInc incv = vgpr_increment(sd.inc);
+  uint32_t head = 0, cnt = 0;
+  for (uint32_t step = 1; step <= a.steps; ++step) {
+    queue_fill<Src, kRegSlots, kMultiBlock>(src, g, incv, &zig, q, head, cnt, D);
+    double z[D];
+#pragma unroll
+    for (uint32_t k = 0; k < D; ++k) {
+      const double v = q[(head + k) & (kRegSlots - 1)][threadIdx.x];
+      z[k] = mirror ? -v : v;  // sample_into (normal.rs:70-74)
+    }
+    head = (head + D) & (kRegSlots - 1);
+    cnt -= D;
+#pragma unroll
+    for (uint32_t i = 0; i < D; ++i) {
+      double eps = 0.0;
+#pragma unroll
+      for (uint32_t k = 0; k <= i; ++k) eps += lt[i * (i + 1) / 2 + k] * z[k];
+      s[i] = s[i] * exp(dr[i] + df[i] * eps);
+      tile[tpos + i][threadIdx.x] = s[i];
+    }
+    tpos += D;
+    if (tpos + D > kCap || step == a.steps) flush_tile(a, tile, tpos, col0);
+  }
+}
+
 int run_multi_gbm_paths(const double* s0, const double* mu, const double* sigma,
                         const double* corr, uint64_t d, double t, uint64_t steps,
                         uint64_t n_paths, uint64_t seed, int antithetic, double* out, int where,
@@ -172,10 +259,11 @@ int run_multi_gbm_paths(const double* s0, const double* mu, const double* sigma,
   const size_t out_bytes = sizeof(double) * n_paths * (steps + 1) * d;
   const size_t o_l = where == STOCHA_DEVICE ? 0 : up(out_bytes);
   const size_t o_c = o_l + up(sizeof(double) * d * d);
+  const uint64_t zs_len = d <= kRegD ? 0 : d * stride;  // per-row z and s in registers
   const size_t o_z = o_c + up(sizeof(double) * 3 * d);
-  const size_t o_s = o_z + up(sizeof(double) * d * stride);
+  const size_t o_s = o_z + up(sizeof(double) * zs_len);
   Workspace* ws = nullptr;
-  if (int rc = workspace(o_s + up(sizeof(double) * d * stride), 0, &ws, err)) return rc;
+  if (int rc = workspace(o_s + up(sizeof(double) * zs_len), 0, &ws, err)) return rc;
   char* wb = static_cast<char*>(ws->dev);
   double* target = where == STOCHA_DEVICE ? out : reinterpret_cast<double*>(wb);
   MultiArgs a;
@@ -197,7 +285,16 @@ int run_multi_gbm_paths(const double* s0, const double* mu, const double* sigma,
   if (blocks > 0x7fffffffull) return err.set(STOCHA_EINVAL, "n_paths too large for one launch");
   KernelTimer timer(st);
   timer.start();
-  if (d <= (uint64_t)kMultiTile)
+  const dim3 grid((unsigned)blocks), blk(kMultiBlock);
+  if (d == 1) multi_reg_kernel<1><<<grid, blk, 0, st>>>(a, sd);
+  else if (d == 2) multi_reg_kernel<2><<<grid, blk, 0, st>>>(a, sd);
+  else if (d == 3) multi_reg_kernel<3><<<grid, blk, 0, st>>>(a, sd);
+  else if (d == 4) multi_reg_kernel<4><<<grid, blk, 0, st>>>(a, sd);
+  else if (d == 5) multi_reg_kernel<5><<<grid, blk, 0, st>>>(a, sd);
+  else if (d == 6) multi_reg_kernel<6><<<grid, blk, 0, st>>>(a, sd);
+  else if (d == 7) multi_reg_kernel<7><<<grid, blk, 0, st>>>(a, sd);
+  else if (d == 8) multi_reg_kernel<8><<<grid, blk, 0, st>>>(a, sd);
+  else if (d <= (uint64_t)kMultiTile)
     multi_feed_kernel<true><<<(unsigned)blocks, kMultiBlock, 0, st>>>(a, sd);
   else
     multi_feed_kernel<false><<<(unsigned)blocks, kMultiBlock, 0, st>>>(a, sd);

@@ -8,7 +8,9 @@
 // so a thread's consecutive values are 8 bytes apart and neighbouring threads
 // are a whole row apart.  To make the HBM writes coalesced the block stages
 // kTile columns of its 256 rows in LDS and then writes them row by row: 8
-// consecutive lanes cover one 64-byte row segment.
+// consecutive lanes cover one 64-byte row segment.  Thread t owns tile column
+// t % kTile and rows t / kTile + kTileRows * j, so a flush is kTile passes of
+// one LDS read, one store and one pointer add.
 #include "host.hpp"
 #include "drivers.hpp"
 
@@ -16,6 +18,7 @@ namespace stocha {
 
 constexpr int kPathBlock = 256;
 constexpr int kTile = 8;
+constexpr int kTileRows = kPathBlock / kTile;  // rows written per pass of a flush
 
 // Step hook: column step+1 of the block's rows goes to the LDS tile; a full
 // tile (or the last column) is written out row-major.  Runs on every thread
@@ -27,12 +30,12 @@ struct TileHook {
   // (LDS-only barriers: the row stores stay in flight across them)
   __device__ __forceinline__ void flush(uint64_t c0, int cols) const {
     lds_barrier();
-    // element e of the tile is (row e / cols, col e % cols)
-    const int n_el = (int)rows_here * cols;
-    for (int e = threadIdx.x; e < n_el; e += kPathBlock) {
-      const int r = e / cols;
-      const int c = e - r * cols;
-      out[(row0 + r) * width + c0 + c] = tile[c][r];
+    const int c = (int)threadIdx.x % kTile;
+    const uint32_t r0 = threadIdx.x / kTile;
+    if (c < cols) {
+      double* p = out + (row0 + r0) * width + c0 + c;
+      const uint64_t pass = (uint64_t)kTileRows * width;
+      for (uint32_t r = r0; r < (uint32_t)rows_here; r += kTileRows, p += pass) *p = tile[c][r];
     }
     lds_barrier();
   }
