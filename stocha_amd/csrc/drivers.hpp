@@ -19,6 +19,16 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders this wavefront's LDS accesses across its lanes: the LDS performs a
+// wavefront's operations in issue order, so a read issued after another
+// lane's write of the same wavefront sees it; only the compiler must keep
+// the order (nothing is emitted).  For tiles that one wavefront writes and
+// reads back, with no block barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
+  asm volatile("" ::: "memory");
+}
+
 // Step-input sources for queue_path.  A source turns the lanes' draws, one
 // per lane per call, into queue entries; Src::kPerStep entries make one
 // step's input.  Sources are wave-collective: feed(u, z, rec, act) is called
@@ -44,6 +54,13 @@ struct NormalSrc {
   __device__ __forceinline__ uint64_t pending_mask() const { return zf.pending_mask(); }
   __device__ __forceinline__ uint64_t feed(uint64_t u, const Tables* z, Rec& r, uint64_t act) {
     return zf.feed(u, z, r, act);
+  }
+  // (queue_fill_ahead: u's fast-table entry read by the caller)
+  using Wave = ZigWave<kFp32>;
+  using Fast = typename Wave::Fast;
+  __device__ __forceinline__ uint64_t feed_with(uint64_t u, const Fast f, const Tables* z, Rec& r,
+                                                uint64_t act) {
+    return zf.feed_with(u, f, z, r, act);
   }
   template <uint32_t kSlots, int kCols, class M>
   __device__ static __forceinline__ void apply(M& m, const Rec (*q)[kCols], uint32_t head) {
@@ -268,6 +285,63 @@ __device__ __forceinline__ void queue_fill(Src& src, Pcg& g, const LcgInc& incv,
   } while (need != 0);
 }
 
+// The generator one draw ahead, for kernels with few waves per SIMD, where
+// the fast-table read of each draw would stall the fill loop: u is the DXSM
+// output of g's state (the lane's next draw), f its fast-table entry (read
+// one fill iteration before it is used), h1m0 the product (s.hi >> 32) * M.lo
+// that the output and the next step share.  A lane's draws are the same
+// values in the same order as with queue_fill.
+template <class Src>
+struct DrawAhead {
+  Pcg g;
+  uint64_t u;
+  uint32_t h1m0;
+  typename Src::Fast f;
+  __device__ __forceinline__ DrawAhead(const U128 s, const typename Src::Tables* zig) : g{s} {
+    h1m0 = (uint32_t)(g.s.hi >> 32) * (uint32_t)kPcgMul;
+    u = g.dxsm(h1m0);
+    f = Src::Wave::fast_entry(zig, u);
+  }
+};
+
+// queue_fill with the generator one draw ahead: each iteration steps the
+// active lanes' states, forms the next draw and issues its table read, then
+// feeds the current draw with the entry read one iteration earlier.
+// Inactive lanes keep their state, so their next draw (and entry) is
+// recomputed unchanged.
+template <class Src, uint32_t kSlots, int kCols>
+__device__ __forceinline__ void queue_fill_ahead(Src& src, DrawAhead<Src>& da,
+                                                 const LcgInc& incv,
+                                                 const typename Src::Tables* zig,
+                                                 typename Src::Rec (*q)[kCols], uint32_t head,
+                                                 uint32_t& cnt, uint32_t need_n) {
+  static_assert((kSlots & (kSlots - 1)) == 0, "ring size");
+  constexpr uint32_t kCap = kSlots - 1;
+  using Rec = typename Src::Rec;
+  uint64_t need = lanes(cnt < need_n);
+  do {
+    uint64_t act = lanes(cnt < kCap);
+    if constexpr (Src::kDefer > 0) {
+      const uint64_t waiting = src.pending_mask();
+      const uint32_t n_wait = (uint32_t)__builtin_popcount((uint32_t)waiting) +
+                              (uint32_t)__builtin_popcount((uint32_t)(waiting >> 32));
+      const bool go = (n_wait >= Src::kDefer) | ((waiting & need) != 0);
+      act &= go ? ~0ull : ~waiting;
+    }
+    const uint64_t u = da.u;
+    const typename Src::Fast f = da.f;
+    da.g.step_masked(incv.lo, incv.hi, act, da.h1m0);
+    da.h1m0 = (uint32_t)(da.g.s.hi >> 32) * (uint32_t)kPcgMul;
+    da.u = da.g.dxsm(da.h1m0);
+    da.f = Src::Wave::fast_entry(zig, da.u);
+    Rec r;
+    const uint64_t got = src.feed_with(u, f, zig, r, act);
+    q[(head + cnt) & (kSlots - 1)][threadIdx.x] = r;
+    cnt = add_lanes(cnt, got);
+    need = lanes(cnt < need_n);
+  } while (need != 0);
+}
+
 // The loop itself, on a caller-owned model (for models whose state the
 // caller reads afterwards).
 template <class Src, uint32_t kSlots, int kCols, class Model, class Hook = NoHook>
@@ -280,6 +354,26 @@ __device__ __forceinline__ void queue_run(Src& src, Model& m, Pcg& g, const U128
   uint32_t head = 0, cnt = 0;
   for (uint32_t step = 0; step < steps; ++step) {
     queue_fill<Src, kSlots, kCols>(src, g, incv, zig, q, head, cnt, kNeed);
+    Src::template apply<kSlots, kCols>(m, q, head);
+    head = (head + kNeed) & (kSlots - 1);
+    cnt -= kNeed;
+    hook(step, m);
+  }
+}
+
+// queue_run with the generator one draw ahead (queue_fill_ahead).
+template <class Src, uint32_t kSlots, int kCols, class Model, class Hook = NoHook>
+__device__ __forceinline__ void queue_run_ahead(Src& src, Model& m, const U128 s0, const U128 inc,
+                                                uint32_t steps, const typename Src::Tables* zig,
+                                                typename Src::Rec (*q)[kCols],
+                                                Hook hook = Hook()) {
+  static_assert((kSlots & (kSlots - 1)) == 0 && kSlots > Src::kPerStep, "ring size");
+  constexpr uint32_t kNeed = Src::kPerStep;
+  const LcgInc incv = vgpr_increment(inc);
+  DrawAhead<Src> da(s0, zig);
+  uint32_t head = 0, cnt = 0;
+  for (uint32_t step = 0; step < steps; ++step) {
+    queue_fill_ahead<Src, kSlots, kCols>(src, da, incv, zig, q, head, cnt, kNeed);
     Src::template apply<kSlots, kCols>(m, q, head);
     head = (head + kNeed) & (kSlots - 1);
     cnt -= kNeed;

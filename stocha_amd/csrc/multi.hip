@@ -63,37 +63,38 @@ struct MultiArgs {
   uint32_t d, steps;
 };
 
-// Writes the tile's tpos values of each of the block's rows (output columns
-// col0 .. col0+tpos-1) and empties the tile.  Every thread of the block calls
-// it together.  Thread t owns tile column t % kMultiTile and rows t /
-// kMultiTile + kFlushRows * j: each pass stores kFlushRows row segments of up
-// to kMultiTile consecutive values (16 lanes per 128-byte segment), one
-// pointer add per store.  (LDS-only barriers: the row stores stay in flight
-// across them.)
-constexpr uint32_t kFlushRows = kMultiBlock / kMultiTile;
-static_assert(kMultiBlock % kMultiTile == 0, "flush layout");
+// Writes the tile's tpos values of each of the wavefront's rows (output
+// columns col0 .. col0+tpos-1) and empties the tile; every lane of the
+// wavefront calls it together.  Each wavefront owns its 64 rows of the tile
+// (no block barrier).  Lane l owns tile column l % kMultiTile and rows
+// l / kMultiTile + kFlushRows * j: each pass stores kFlushRows row segments
+// of up to kMultiTile consecutive values (16 lanes per 128-byte segment), one
+// pointer add per store.
+constexpr uint32_t kFlushRows = 64 / kMultiTile;
+static_assert(64 % kMultiTile == 0, "flush layout");
 __device__ __forceinline__ void flush_tile(const MultiArgs& a, double (*tile)[kMultiBlock + 1],
                                            uint32_t& tpos, uint64_t& col0) {
-  lds_barrier();
-  const uint64_t row0 = (uint64_t)blockIdx.x * kMultiBlock;
-  const uint64_t left = a.n_rows - row0;
-  const uint32_t rows_here = left < (uint64_t)kMultiBlock ? (uint32_t)left : kMultiBlock;
+  wave_lds_sync();
+  const uint32_t wbase = threadIdx.x & ~63u, lane = threadIdx.x & 63u;
+  const uint64_t row0 = (uint64_t)blockIdx.x * kMultiBlock + wbase;
+  const uint32_t rows_here =
+      a.n_rows <= row0 ? 0u : a.n_rows - row0 < 64u ? (uint32_t)(a.n_rows - row0) : 64u;
   const uint64_t width = (a.steps + 1ull) * a.d;
-  const uint32_t c = threadIdx.x % kMultiTile;
-  const uint32_t r0 = threadIdx.x / kMultiTile;
+  const uint32_t c = lane % kMultiTile;
+  const uint32_t r0 = lane / kMultiTile;
   if (c < tpos) {
     double* p = a.out + (row0 + r0) * width + col0 + c;
     const uint64_t pass = (uint64_t)kFlushRows * width;
-    for (uint32_t r = r0; r < rows_here; r += kFlushRows, p += pass) *p = tile[c][r];
+    for (uint32_t r = r0; r < rows_here; r += kFlushRows, p += pass) *p = tile[c][wbase + r];
   }
-  lds_barrier();
+  wave_lds_sync();
   col0 += tpos;
   tpos = 0;
 }
 
 // The queue model: one normal per apply; the d-th completes a step
-// (multi_gbm.rs:60-76).  Every lane of the block completes its steps on the
-// same queue steps, so the tile flushes' barriers line up.
+// (multi_gbm.rs:60-76).  Every lane of a wavefront completes its steps on the
+// same queue steps, so the wavefront flushes its tile rows together.
 template <bool kTiled>
 struct MultiModel {
   const MultiArgs a;
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(kMultiBlock) void multi_feed_kernel(const MultiArgs
   Pcg g{stream_at(&sd, path)};
   MultiModel<kTiled> m(a, row, mirror, valid, tile);
   __syncthreads();
+  if ((uint64_t)blockIdx.x * kMultiBlock + (threadIdx.x & ~63u) >= a.n_rows) return;  // no row
   Src src(a);
   queue_run<Src, 8, kMultiBlock>(src, m, g, sd.inc, a.steps * a.d, &zig, q);
 }
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kMultiBlock) void multi_reg_kernel(const MultiArgs 
   const bool valid = row < a.n_rows;
   const bool mirror = valid && row >= a.base_rows;
   const uint64_t path = !valid ? 0 : (mirror ? row - a.base_rows : row);
-  Pcg g{stream_at(&sd, path)};
+  const U128 st0 = stream_at(&sd, path);
   double s[D], lt[D * (D + 1) / 2], dr[D], df[D];
 #pragma unroll
   for (uint32_t i = 0; i < D; ++i) {
@@ -207,11 +209,13 @@ __global__ __launch_bounds__(kMultiBlock) void multi_reg_kernel(const MultiArgs 
   uint32_t tpos = D;
   uint64_t col0 = 0;
   __syncthreads();
+  if ((uint64_t)blockIdx.x * kMultiBlock + (threadIdx.x & ~63u) >= a.n_rows) return;  // no row
   Src src(a);
   const LcgInc incv = vgpr_increment(sd.inc);
+  DrawAhead<Src> da(st0, &zig);
   uint32_t head = 0, cnt = 0;
   for (uint32_t step = 1; step <= a.steps; ++step) {
-    queue_fill<Src, kRegSlots, kMultiBlock>(src, g, incv, &zig, q, head, cnt, D);
+    queue_fill_ahead<Src, kRegSlots, kMultiBlock>(src, da, incv, &zig, q, head, cnt, D);
     double z[D];
 #pragma unroll
     for (uint32_t k = 0; k < D; ++k) {

@@ -6,11 +6,13 @@
 // (drivers.hpp: per-lane ziggurat state machine, steps applied by the whole
 // wave).  The output is the reference's row-major (n_rows, steps+1) matrix,
 // so a thread's consecutive values are 8 bytes apart and neighbouring threads
-// are a whole row apart.  To make the HBM writes coalesced the block stages
-// kTile columns of its 256 rows in LDS and then writes them row by row: 8
-// consecutive lanes cover one 64-byte row segment.  Thread t owns tile column
-// t % kTile and rows t / kTile + kTileRows * j, so a flush is kTile passes of
-// one LDS read, one store and one pointer add.
+// are a whole row apart.  To make the HBM writes coalesced each wavefront
+// stages kTile columns of its 64 rows in LDS and then writes them row by row:
+// 8 consecutive lanes cover one 64-byte row segment.  Lane l owns tile column
+// l % kTile and rows l / kTile + kWaveRows * j, so a flush is kTile passes of
+// one LDS read, one store and one pointer add.  The tile of a wavefront is
+// its own (no block barrier: the wavefronts of a block run their steps
+// independently).
 #include "host.hpp"
 #include "drivers.hpp"
 
@@ -18,26 +20,27 @@ namespace stocha {
 
 constexpr int kPathBlock = 256;
 constexpr int kTile = 8;
-constexpr int kTileRows = kPathBlock / kTile;  // rows written per pass of a flush
+constexpr int kWaveRows = 64 / kTile;  // rows written per pass of a flush
 
-// Step hook: column step+1 of the block's rows goes to the LDS tile; a full
-// tile (or the last column) is written out row-major.  Runs on every thread
-// of the block after each step, so its barriers line up.
+// Step hook: column step+1 of the wavefront's rows goes to the LDS tile; a
+// full tile (or the last column) is written out row-major.  Runs on every
+// lane of the wavefront after each step.
 struct TileHook {
   double (*tile)[kPathBlock + 1];
   double* out;
-  uint64_t row0, width, rows_here;
-  // (LDS-only barriers: the row stores stay in flight across them)
+  uint64_t row0, width;  // row0: the wavefront's first row
+  uint32_t rows_here;    // the wavefront's rows below n_rows (0..64)
   __device__ __forceinline__ void flush(uint64_t c0, int cols) const {
-    lds_barrier();
-    const int c = (int)threadIdx.x % kTile;
-    const uint32_t r0 = threadIdx.x / kTile;
+    wave_lds_sync();
+    const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+    const int c = (int)(lane % kTile);
+    const uint32_t r0 = lane / kTile;
     if (c < cols) {
       double* p = out + (row0 + r0) * width + c0 + c;
-      const uint64_t pass = (uint64_t)kTileRows * width;
-      for (uint32_t r = r0; r < (uint32_t)rows_here; r += kTileRows, p += pass) *p = tile[c][r];
+      const uint64_t pass = (uint64_t)kWaveRows * width;
+      for (uint32_t r = r0; r < rows_here; r += kWaveRows, p += pass) *p = tile[c][wbase + r];
     }
-    lds_barrier();
+    wave_lds_sync();
   }
   template <class M>
   __device__ __forceinline__ void operator()(uint32_t step, const M& m) const {
@@ -48,7 +51,10 @@ struct TileHook {
   }
 };
 
-template <class Src, class Model, class Const, uint32_t kSlots>
+// kAhead: the fill runs the generator one draw ahead (queue_run_ahead); for
+// the normal-only sources, whose table read per draw is the fill loop's
+// longest wait at these few waves per SIMD.
+template <class Src, class Model, class Const, uint32_t kSlots, bool kAhead = false>
 __global__ __launch_bounds__(kPathBlock) void paths_kernel(const Const mc, const StreamDesc sd,
                                                            const uint64_t n_rows,
                                                            const uint64_t base_rows,
@@ -62,22 +68,32 @@ __global__ __launch_bounds__(kPathBlock) void paths_kernel(const Const mc, const
 
   const uint64_t row0 = (uint64_t)blockIdx.x * kPathBlock;
   const uint64_t row = row0 + threadIdx.x;
-  // Threads past the last row still take part in the steps and barriers;
-  // they step a valid stream (path 0) and their values are never written.
+  // Lanes past the last row in a wavefront with rows still take part in the
+  // steps; they step a valid stream (path 0) and their values are never
+  // written.  Wavefronts with no row leave after the table load.
   const bool valid = row < n_rows;
   const bool mirror = valid && row >= base_rows;
   const uint64_t path = !valid ? 0 : (mirror ? row - base_rows : row);
-  Pcg g{stream_at(&sd, path)};
+  const U128 s0 = stream_at(&sd, path);
   tile[0][threadIdx.x] = mc.s0;
   __syncthreads();
 
   TileHook hook;
   hook.tile = tile;
   hook.out = out;
-  hook.row0 = row0;
+  hook.row0 = row0 + (threadIdx.x & ~63u);
   hook.width = (uint64_t)steps + 1;
-  hook.rows_here = n_rows - row0 < (uint64_t)kPathBlock ? n_rows - row0 : kPathBlock;
-  (void)queue_path<Src, Model, kSlots, kPathBlock>(mc, g, sd.inc, steps, &zig, q, mirror, hook);
+  hook.rows_here = n_rows <= hook.row0 ? 0u
+                   : n_rows - hook.row0 < 64u ? (uint32_t)(n_rows - hook.row0) : 64u;
+  if (hook.rows_here == 0) return;  // wave-uniform
+  if constexpr (kAhead) {
+    Src src(mc);
+    Model m(mc, mirror);
+    queue_run_ahead<Src, kSlots, kPathBlock>(src, m, s0, sd.inc, steps, &zig, q, hook);
+  } else {
+    Pcg g{s0};
+    (void)queue_path<Src, Model, kSlots, kPathBlock>(mc, g, sd.inc, steps, &zig, q, mirror, hook);
+  }
 }
 
 int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n_rows,
@@ -92,11 +108,11 @@ int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n
   const uint32_t st32 = (uint32_t)steps;
   switch (model) {
     case kGbm:
-      paths_kernel<GbmSrc<false>, Gbm64, GbmConst, 8><<<grid, block, 0, st>>>(
+      paths_kernel<GbmSrc<false>, Gbm64, GbmConst, 8, true><<<grid, block, 0, st>>>(
           *(const GbmConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     case kHeston:
-      paths_kernel<HestonSrc<false>, Heston64, HestonConst, 8><<<grid, block, 0, st>>>(
+      paths_kernel<HestonSrc<false>, Heston64, HestonConst, 8, true><<<grid, block, 0, st>>>(
           *(const HestonConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     case kMerton:
@@ -104,7 +120,7 @@ int launch_paths(int model, const void* mconst, const StreamDesc& sd, uint64_t n
           *(const MertonConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     case kHullWhite:
-      paths_kernel<GbmSrc<false>, HullWhite64, HullWhiteConst, 8><<<grid, block, 0, st>>>(
+      paths_kernel<GbmSrc<false>, HullWhite64, HullWhiteConst, 8, true><<<grid, block, 0, st>>>(
           *(const HullWhiteConst*)mconst, sd, n_rows, base_rows, st32, d_out);
       break;
     case kHestonQe:

@@ -126,18 +126,34 @@ struct Pcg {
   // before the block ends; the block sets SCC (s_and_saveexec), and says so.
   __device__ __forceinline__ uint64_t draw_step(const uint64_t inc_lo, const uint64_t inc_hi,
                                                 const uint64_t act) {
+    const uint32_t h1m0 = (uint32_t)(s.hi >> 32) * (uint32_t)kPcgMul;
+    const uint64_t out = dxsm(h1m0);
+    step_masked(inc_lo, inc_hi, act, h1m0);
+    return out;
+  }
+
+  // DXSM (rand_pcg output_dxsm): hi ^= hi >> 32; hi *= M; hi ^= hi >> 48;
+  // hi *= lo | 1, on 32-bit limbs; h1m0 = (s.hi >> 32) * M.lo (mod 2^32).
+  __device__ __forceinline__ uint64_t dxsm(const uint32_t h1m0) const {
     constexpr uint32_t m0 = (uint32_t)kPcgMul, m1 = (uint32_t)(kPcgMul >> 32);
-    uint32_t l0 = (uint32_t)s.lo, l1 = (uint32_t)(s.lo >> 32);
-    uint32_t h0 = (uint32_t)s.hi, h1 = (uint32_t)(s.hi >> 32);
-    // DXSM (rand_pcg output_dxsm): hi ^= hi >> 32; hi *= M; hi ^= hi >> 48; hi *= lo | 1
-    const uint32_t h1m0 = h1 * m0;
+    const uint32_t l0 = (uint32_t)s.lo, l1 = (uint32_t)(s.lo >> 32);
+    const uint32_t h0 = (uint32_t)s.hi, h1 = (uint32_t)(s.hi >> 32);
     const uint32_t x0 = h0 ^ h1;
     const uint64_t p = (uint64_t)x0 * m0;
     const uint32_t y1 = (uint32_t)(p >> 32) + x0 * m1 + h1m0;
     const uint32_t y0 = (uint32_t)p ^ (y1 >> 16);
     const uint32_t o0 = l0 | 1u;
     const uint64_t q = (uint64_t)y0 * o0;
-    const uint64_t out = ((uint64_t)((uint32_t)(q >> 32) + y0 * l1 + y1 * o0) << 32) | (uint32_t)q;
+    return ((uint64_t)((uint32_t)(q >> 32) + y0 * l1 + y1 * o0) << 32) | (uint32_t)q;
+  }
+
+  // The LCG step s * M + inc on the lanes of `act` (see draw_step);
+  // h1m0 = (s.hi >> 32) * M.lo of the current state.
+  __device__ __forceinline__ void step_masked(const uint64_t inc_lo, const uint64_t inc_hi,
+                                              const uint64_t act, const uint32_t h1m0) {
+    constexpr uint32_t m0 = (uint32_t)kPcgMul, m1 = (uint32_t)(kPcgMul >> 32);
+    uint32_t l0 = (uint32_t)s.lo, l1 = (uint32_t)(s.lo >> 32);
+    uint32_t h0 = (uint32_t)s.hi, h1 = (uint32_t)(s.hi >> 32);
     uint64_t A, D, Q, cA, cD, dm;
     uint32_t cr;
     asm volatile(
@@ -177,7 +193,6 @@ struct Pcg {
     (void)dm2;
     s.lo = ((uint64_t)l1 << 32) | l0;
     s.hi = ((uint64_t)h1 << 32) | h0;
-    return out;
   }
 
   // pcg64.rs:37-39: (u >> 11) * 2^-53, exact

@@ -156,20 +156,31 @@ struct ZigWave {
 
   __device__ __forceinline__ uint64_t pending_mask() const { return missed | tail1 | tail2; }
 
+  // The fast-path table entry of draw u (layer and sign: bits 0..8).
+  using Fast = typename std::conditional<kFp32, ZigFast32, ZigFast>::type;
+  __device__ static __forceinline__ Fast fast_entry(const Tables* __restrict__ z, uint64_t u) {
+    if constexpr (kFp32) {
+      return z->fast32[(uint32_t)u & 0x1ffu];
+    } else {
+      return z->fast[(uint32_t)u & 0x1ffu];
+    }
+  }
+
   // Consumes draw u on the lanes of `act`; returns the lanes whose draw
   // completed a normal (in out).  `out` is written on every lane.
   __device__ __forceinline__ uint64_t feed(uint64_t u, const Tables* __restrict__ z, Real& out,
                                            uint64_t act) {
-    const uint32_t i = (uint32_t)u & 0x1ffu;
+    return feed_with(u, fast_entry(z, u), z, out, act);
+  }
+  // The same with u's table entry f already read.
+  __device__ __forceinline__ uint64_t feed_with(uint64_t u, const Fast f,
+                                                const Tables* __restrict__ z, Real& out,
+                                                uint64_t act) {
     const uint32_t hi = (uint32_t)(u >> 32);
-    uint32_t th;
+    const uint32_t th = f.th;
     if constexpr (kFp32) {
-      const ZigFast32 f = z->fast32[i];
-      th = f.th;
       out = hi32_to_f32(u) * f.xs;  // the table folds the sign bit (bit 8) into X
     } else {
-      const ZigFast f = z->fast[i];
-      th = f.th;
       out = ((double)(u >> 11) * 0x1p-53) * f.x;
     }
     const uint64_t lt = lanes(hi < th), eq = lanes(hi == th);
