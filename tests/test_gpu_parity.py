@@ -62,8 +62,13 @@ def test_normals_match_oracle(gpu, orc, prec):
 
 # -------------------------------------------------------------------- paths --
 @pytest.mark.parametrize("n_paths,steps,anti", [(1000, 37, False), (2048, 252, False),
-                                                (999, 50, True), (1, 1, False)])
+                                                (999, 50, True), (1, 1, False),
+                                                (64, 9, False), (65, 8, True),
+                                                (193, 16, False), (257, 7, True)])
 def test_gbm_paths(gpu, orc, n_paths, steps, anti):
+    # each wavefront writes its own rows: 64 / 65 / 193 / 257 rows leave whole
+    # wavefronts without rows and one ragged wavefront; 8 / 16 steps end on a
+    # full 8-column tile, 7 / 9 on a partial one
     kw = dict(s0=100.0, mu=0.05, sigma=0.2, t=1.0, steps=steps, n_paths=n_paths, seed=42,
               antithetic=anti, q=0.01)
     dev = gpu.gbm(**kw)
@@ -73,7 +78,7 @@ def test_gbm_paths(gpu, orc, n_paths, steps, anti):
     np.testing.assert_allclose(dev, ref, rtol=RTOL64, atol=0)
 
 
-@pytest.mark.parametrize("n_paths,steps", [(2000, 252), (300, 17)])
+@pytest.mark.parametrize("n_paths,steps", [(2000, 252), (300, 17), (64, 9), (130, 16)])
 def test_heston_paths(gpu, orc, n_paths, steps):
     kw = dict(s0=100.0, v0=0.04, mu=0.05, kappa=2.0, theta=0.04, xi=0.3, rho=-0.7, t=1.0,
               steps=steps, n_paths=n_paths, seed=42)
@@ -116,12 +121,17 @@ def _corr(d, seed):
     return c / np.outer(dg, dg), rng
 
 
-@pytest.mark.parametrize("d,n,steps,anti", [(5, 301, 37, False), (8, 1000, 20, True),
+@pytest.mark.parametrize("d,n,steps,anti", [(2, 190, 41, True), (4, 65, 33, False),
+                                            (5, 301, 37, False), (6, 777, 17, True),
+                                            (7, 64, 23, False), (8, 1000, 20, True),
                                             (12, 257, 9, False), (16, 513, 11, True),
                                             (17, 300, 6, False), (24, 129, 5, True)])
 def test_multi_gbm_tile_shapes(gpu, orc, d, n, steps, anti):
-    # d <= 16 stages whole steps in the LDS tile (d = 12, 16: one step per
-    # flush; 5, 8: several); d > 16 writes rows directly; ragged last blocks
+    # d <= 8: one register kernel per d (each instantiation here or in the
+    # tests above); d <= 16 stages whole steps in the LDS tile (d = 12, 16:
+    # one step per flush; 2 to 8: several, 7: a 14-value tile); d > 16 writes
+    # rows directly; ragged last wavefronts and blocks, and n = 64: one
+    # wavefront with rows, three without
     corr, rng = _corr(d, d)
     kw = dict(s0=list(rng.uniform(50, 150, d)), mu=list(rng.uniform(0.0, 0.08, d)),
               sigma=list(rng.uniform(0.1, 0.4, d)), corr=corr, t=1.0, steps=steps, n_paths=n,
