@@ -92,6 +92,20 @@ struct Gbm64 {
   __device__ __forceinline__ double terminal() const { return s; }
 };
 
+// max(x, 0) as one v_max instruction: fmax's lowering first quiets x with a
+// second v_max.  Same result for every x (v_max returns 0 for a NaN x, as
+// fmax does).
+__device__ __forceinline__ float max0(float x) {
+  float r;
+  asm("v_max_f32 %0, 0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+__device__ __forceinline__ double max0(double x) {
+  double r;
+  asm("v_max_f64 %0, 0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 struct Heston64 {
   static constexpr bool kFp32 = false;
   HestonConst c;
@@ -99,7 +113,7 @@ struct Heston64 {
   __device__ __forceinline__ Heston64(const HestonConst& k, bool) : c(k), s(k.s0), v(k.v0) {}
   __device__ __forceinline__ void apply(double dw1, double dz) {
     double dw2 = c.rho * dw1 + c.rho_comp * dz;
-    double v_plus = fmax(v, 0.0);
+    double v_plus = max0(v);
     double sqrt_v = sqrt(v_plus);
     v += c.kappa * (c.theta - v_plus) * c.dt + c.xi * sqrt_v * c.sqrt_dt * dw2;
     s *= exp((c.mu - 0.5 * v_plus) * c.dt + sqrt_v * c.sqrt_dt * dw1);
@@ -209,7 +223,7 @@ struct Heston32 {
   __device__ __forceinline__ Heston32(const HestonConst& k, bool) : c(k), ls(0.f), v(k.v0_f) {}
   __device__ __forceinline__ void apply(float dw1, float dz) {
     float dw2 = fmaf(c.rho_f, dw1, c.rho_comp_f * dz);
-    float v_plus = fmaxf(v, 0.f);
+    float v_plus = max0(v);
     // v_sqrt_f32 (1 ulp); the FP32 form's error bar is 1e-4 of the FP64 price
     float sqrt_v = __builtin_amdgcn_sqrtf(v_plus);
     // v += kappa*(theta - v+)*dt + xi*sqrt(v+)*sqrt(dt)*dw2
