@@ -237,18 +237,17 @@ def european_price(model: str, params: dict, t: float, steps: int, n_paths: int,
     k = np.ascontiguousarray([_S._f64(v, "strikes") for v in strikes], dtype=np.float64)
     steps, n_paths, seed = _S._uint(steps, "steps"), _S._uint(n_paths, "n_paths"), \
         _S._uint(seed, "seed")
-    lib, eb = _lib.load(), _lib.ErrBuf()
+    lib, eb = _lib.load(), _lib.errbuf()
     ns = len(k)
-    price = np.zeros(max(ns, 1))
-    se = np.zeros(max(ns, 1))
-    sums = np.zeros(2 * max(ns, 1))
-    rc = lib.stocha_european_fused(mid, _S._pd(p), len(p), _S._f64(t, "t"), steps, n_paths,
-                                   seed, int(precision), sched, _S._pd(k), ns,
-                                   _S._f64(r, "r"),
-                                   _S._pd(price), _S._pd(se), _S._pd(sums), eb.buf, eb.n)
+    m = max(ns, 1)
+    out = np.zeros(4 * m)  # price | std_err | (sum, sumsq) pairs
+    a = out.ctypes.data  # raw addresses: cheaper per call than ctypes pointer objects
+    rc = lib.stocha_european_fused(mid, p.ctypes.data, len(p), _S._f64(t, "t"), steps, n_paths,
+                                   seed, int(precision), sched, k.ctypes.data, ns,
+                                   _S._f64(r, "r"), a, a + 8 * m, a + 16 * m, eb.buf, eb.n)
     _lib.check(rc, eb)
-    return {"price": price[:ns], "std_err": se[:ns], "sum": sums[0:2 * ns:2],
-            "sumsq": sums[1:2 * ns:2]}
+    return {"price": out[:ns], "std_err": out[m:m + ns], "sum": out[2 * m:2 * m + 2 * ns:2],
+            "sumsq": out[2 * m + 1:2 * m + 2 * ns:2]}
 
 
 def lsmc_american_option_ex(s0: float, k: float, r: float, sigma: float, t: float, steps: int,

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -75,7 +76,7 @@ _SIGS = {
     "stocha_hull_white": ([_d] * 5 + [_u, _u, _u, _vp, _i, _ps, _sz], C.c_int),
     "stocha_gaussian_copula": ([_pd, _u, _u, _u, _u, _vp, _i, _ps, _sz], C.c_int),
     "stocha_student_t_copula": ([_pd, _u, _u, _d, _u, _u, _vp, _i, _ps, _sz], C.c_int),
-    "stocha_european_fused": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _pd, _pd, _pd,
+    "stocha_european_fused": ([_i, _vp, _u, _d, _u, _u, _u, _i, _i, _vp, _u, _d, _vp, _vp, _vp,
                                _ps, _sz], C.c_int),
     "stocha_european_fused_noalloc": ([_i, _pd, _u, _d, _u, _u, _u, _i, _i, _pd, _u, _d, _vp,
                                        _vp, _vp, _ps, _sz], C.c_int),
@@ -145,6 +146,18 @@ def host_array(shape, dtype=np.float64):
 
 def exported_names():
     return list(_SIGS)
+
+
+_tls = threading.local()
+
+
+def errbuf() -> "ErrBuf":
+    """This thread's reusable error buffer (entry points write it only when
+    they return an error)."""
+    eb = getattr(_tls, "eb", None)
+    if eb is None:
+        eb = _tls.eb = ErrBuf()
+    return eb
 
 
 class ErrBuf:

@@ -152,7 +152,7 @@ def _lsmc(s0, k, r, sigma, t, steps, n_paths, is_put, poly_degree, seed, precisi
     steps, n_paths = _uint(steps, "steps"), _uint(n_paths, "n_paths")
     is_put = _bool(is_put, "is_put")
     poly_degree, seed = _uint(poly_degree, "poly_degree"), _uint(seed, "seed")
-    lib, eb = L.load(), L.ErrBuf()
+    lib, eb = L.load(), L.errbuf()
     price, se = C.c_double(), C.c_double()
     rc = lib.stocha_lsmc_ex(*vals, steps, n_paths, int(is_put), poly_degree, seed, precision,
                             C.byref(price), C.byref(se), eb.buf, eb.n)

@@ -343,6 +343,13 @@ int stocha_timing_enable(int on) {
 
 int stocha_last_kernel_ms(double* ms, uint64_t* launches) {
   std::lock_guard<std::mutex> lk(globals().mu);
+  Globals& g = globals();
+  if (g.timer_pending) {
+    g.timer_pending = false;
+    float t = 0.f;
+    if (hipEventSynchronize(g.ev1) == hipSuccess && hipEventElapsedTime(&t, g.ev0, g.ev1) == hipSuccess)
+      g.last_ms = t;
+  }
   if (ms) *ms = globals().last_ms;
   if (launches) *launches = globals().last_launches;
   return STOCHA_OK;
