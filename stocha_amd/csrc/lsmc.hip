@@ -956,9 +956,11 @@ __global__ __launch_bounds__(kCoopBlock) void lsmc_coop_kernel(const LsmcCoopArg
 template <bool kFp32, int kPpt>
 static int lsmc_coop_launch(const LsmcCoopArgs& args, unsigned grid, hipStream_t st,
                             const Err& err) {
-  int bpc = 0;
-  STOCHA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, lsmc_coop_kernel<kFp32, kPpt>,
-                                                          kCoopBlock, 0));
+  // the kernel's fit, asked once (same code object on every gfx950 device)
+  static int bpc = -1;
+  if (bpc < 0)
+    STOCHA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, lsmc_coop_kernel<kFp32, kPpt>,
+                                                            kCoopBlock, 0));
   if (bpc < 1) return err.set(STOCHA_EDEVICE, "LSMC kernel does not fit one block per CU");
   LsmcCoopArgs a = args;
   void* kargs[] = {&a};
@@ -1033,7 +1035,9 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
     ca.k = k;
     ca.discount = discount;
     ca.part = part;
-    ca.out = scal;
+    // the two totals go straight into the pinned host buffer (device-visible
+    // through unified addressing; read after the stream sync)
+    ca.out = ws->host;
     if (int rc = lsmc_coop<kFp32>(ca, (unsigned)cus, st, err)) return rc;
     STOCHA_HIP(hipGetLastError());
   } else {
@@ -1058,7 +1062,8 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
     STOCHA_HIP(hipGetLastError());
   }
   timer.stop(coop ? 2 : 0);
-  STOCHA_HIP(hipMemcpyAsync(ws->host, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (!coop)
+    STOCHA_HIP(hipMemcpyAsync(ws->host, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   STOCHA_HIP(hipStreamSynchronize(st));
   const double mean = ws->host[0] / (double)n;
   *price = mean;
