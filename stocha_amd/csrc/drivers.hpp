@@ -75,7 +75,10 @@ struct NormalSrc {
     if constexpr (kPer == 1) {
       m.apply(q[head][threadIdx.x]);
     } else {
-      m.apply(q[head][threadIdx.x], q[(head + 1) & (kSlots - 1)][threadIdx.x]);
+      // head advances by 2 from 0 in a ring of an even size, so it is even
+      // and head + 1 needs no wrap (one address, two reads at fixed offsets)
+      static_assert(kSlots % 2 == 0, "ring size");
+      m.apply(q[head][threadIdx.x], q[head + 1][threadIdx.x]);
     }
   }
 };
