@@ -349,6 +349,13 @@ int stocha_stream_u64(uint64_t seed, uint64_t block, uint64_t path0, uint64_t n_
 int stocha_stream_normals(uint64_t seed, uint64_t block, uint64_t path0, uint64_t n_paths,
                           uint64_t n_draws, int precision, double *out, uint64_t *consumed,
                           char *err, size_t errlen);
+/* The same normals drawn on the steppers' queue schedule (lane-mask ziggurat
+ * state machine, per-lane rings, deferred slow block) instead of the
+ * per-normal loop; no reference counterpart (test support: the schedule on
+ * chosen stream positions).  n_draws < 2^32. */
+int stocha_stream_normals_queue(uint64_t seed, uint64_t block, uint64_t path0, uint64_t n_paths,
+                                uint64_t n_draws, int precision, double *out, char *err,
+                                size_t errlen);
 
 #ifdef __cplusplus
 }

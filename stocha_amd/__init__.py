@@ -315,6 +315,19 @@ def stream_normals(seed: int, block: int, path0: int, n_paths: int, n_draws: int
     return out, used
 
 
+def stream_normals_queue(seed: int, block: int, path0: int, n_paths: int, n_draws: int,
+                         precision: int = 64) -> np.ndarray:
+    """The normals of stream_normals drawn on the steppers' queue schedule
+    (same values, another schedule)."""
+    out = np.zeros((n_paths, n_draws), dtype=np.float64)
+    lib, eb = _lib.load(), _lib.errbuf()
+    rc = lib.stocha_stream_normals_queue(seed, block, path0, n_paths, n_draws, int(precision),
+                                         out.ctypes.data_as(_C.POINTER(_C.c_double)), eb.buf,
+                                         eb.n)
+    _lib.check(rc, eb)
+    return out
+
+
 def basket_price(s0: Sequence[float], mu: Sequence[float], sigma: Sequence[float],
                  corr: np.ndarray, weights: Sequence[float], strike: float, r: float, t: float,
                  steps: int, n_paths: int, seed: int = 42, precision: int = 32,

@@ -89,6 +89,7 @@ _SIGS = {
     "stocha_batch_plan_destroy": ([_vp], None),
     "stocha_stream_u64": ([_u, _u, _u, _u, _u, _pu, _ps, _sz], C.c_int),
     "stocha_stream_normals": ([_u, _u, _u, _u, _u, _i, _pd, _pu, _ps, _sz], C.c_int),
+    "stocha_stream_normals_queue": ([_u, _u, _u, _u, _u, _i, _pd, _ps, _sz], C.c_int),
 }
 
 _lib = None

@@ -211,3 +211,21 @@ def test_draw_rate_matches_tables(orc, tables):
     n = 400_000
     _, used = orc.normals(7, 2552, 3, n)
     assert abs(used / n - draws) < 0.002, (used / n, draws)
+
+
+def test_tie_fixture_positions_are_ties(orc, tables):
+    # tests/golden/zig_ties.json (tests/golden/find_zig_ties.c): each listed
+    # draw is the first attempt of its normal and ties the fast test on the
+    # high word, (u >> 32) == T[l] >> 21, with the exact test's outcome as
+    # recorded; the GPU tie tests rely on these positions
+    _, _, _, T = tables
+    fix = json.load(open(os.path.join(HERE, "golden", "zig_ties.json")))
+    seed, block = fix["seed"], fix["block"]
+    assert {t["accepted"] for t in fix["ties"]} == {True, False}
+    for t in fix["ties"]:
+        u = int(orc.raw_u64(seed, block, t["path"], t["draw"] + 1)[t["draw"]])
+        l = u & 0xFF
+        assert (u >> 32) == (T[l] >> 21), t
+        assert ((u >> 11) < T[l]) == t["accepted"], t
+        _, before = orc.normals(seed, block, t["path"], t["normal"])
+        assert before == t["draw"], t
