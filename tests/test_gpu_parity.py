@@ -272,6 +272,23 @@ def test_lsmc_fewest_dates(gpu, orc, steps):
     assert se == pytest.approx(rse, rel=1e-9)
 
 
+@pytest.mark.parametrize("multikernel", [False, True])
+@pytest.mark.parametrize("seed", [0, 1, 4])
+def test_lsmc_dates_at_the_regression_threshold(gpu, orc, monkeypatch, multikernel, seed):
+    # 12-path put books whose dates hold exactly ncols (skipped, lsmc.rs:62)
+    # and ncols + 1 (solved) paths in the money
+    if multikernel:
+        monkeypatch.setenv("STOCHA_LSMC_MULTIKERNEL", "1")
+    kw = dict(s0=100.0, k=92.0, r=0.05, sigma=0.3, t=1.0, steps=20, n_paths=12,
+              is_put=True, poly_degree=3, seed=seed)
+    _, itm, _, _ = orc.lsmc_trace(**kw)
+    assert 4 in [int(m) for m in itm[1:]]
+    p, se = gpu.lsmc_american_option(**kw)
+    rp, rse = orc.lsmc(**kw)
+    assert p == pytest.approx(rp, rel=1e-9)
+    assert se == pytest.approx(rse, rel=1e-9)
+
+
 @pytest.mark.parametrize("n_paths", [262_145, 4_194_304])
 def test_lsmc_fp32_boundaries_agree_with_fp64(gpu, n_paths):
     kw = dict(s0=100.0, k=100.0, r=0.05, sigma=0.2, t=1.0, steps=4, n_paths=n_paths,

@@ -278,3 +278,68 @@ def test_merton_jump_normal_follows_uniform(orc):
             nojumps += 1
         assert paths[i, 1] == s0 * math.exp(ret), i
     assert jumps > 50 and nojumps > 50
+
+
+@pytest.mark.parametrize("above", [False, True])
+def test_merton_jump_test_is_strict(orc, above):
+    """jump_diffusion.rs:80: a jump needs u < lambda*dt, so lambda*dt equal to
+    the drawn uniform gives no jump and the next double above it gives one.
+    With t = steps = 1, dt = 1 and lambda*dt is lambda itself."""
+    s0, mu, sigma, mu_j, sigma_j, t, steps, seed = 100.0, 0.05, 0.2, -0.1, 0.2, 1.0, 1, 11
+    z0, c0 = orc.normals(seed, 1, 0, 1)
+    g = numpy_base_stream(seed)
+    g.advance(int(c0))
+    u = f64(g.random_raw(1)[0])
+    assert 0.0 < u < 1.0
+    lam = math.nextafter(u, 2.0) if above else u
+    p = orc.merton(s0, mu, sigma, lam, mu_j, sigma_j, t, steps, 1, seed=seed)[0]
+    m_bar = math.exp(mu_j + 0.5 * sigma_j * sigma_j) - 1.0
+    ret = (mu - 0.5 * sigma * sigma - lam * m_bar) * 1.0 + sigma * math.sqrt(1.0) * z0[0]
+    if above:
+        zj, _ = orc.normals(seed, 1, int(c0) + 1, 1)
+        ret = ret + (mu_j + sigma_j * zj[0])
+    assert p[1] == s0 * math.exp(ret)
+
+
+# ----------------------------------------------------------------- LSMC --
+def test_lsmc_regression_needs_basis_plus_one_itm_paths(orc):
+    """lsmc.rs:62: a date is regressed only with at least ncols + 1 paths in
+    the money.  Small put books are searched for dates holding exactly ncols
+    and exactly ncols + 1 such paths: the first must be skipped (no
+    coefficients), the second solved."""
+    deg, ncols = 3, 4
+    seen = set()
+    for seed in range(200):
+        coeffs, itm, _, _ = orc.lsmc_trace(100.0, 92.0, 0.05, 0.3, 1.0, 20, 12, True, deg, seed)
+        for step in range(1, 20):
+            m = int(itm[step])
+            if m == ncols:
+                assert np.all(np.isnan(coeffs[step])), (seed, step)
+                seen.add(m)
+            elif m == ncols + 1:
+                assert np.all(np.isfinite(coeffs[step])), (seed, step)
+                seen.add(m)
+        if seen == {ncols, ncols + 1}:
+            return
+    pytest.fail(f"no dates with {ncols} and {ncols + 1} paths in the money found")
+
+
+@pytest.mark.parametrize("is_put,deg", [(True, 3), (False, 2)])
+def test_lsmc_last_date_coefficients_by_numpy(orc, is_put, deg):
+    """lsmc.rs:66-84 at the last regression date, where the cash flow is the
+    discounted terminal payoff (no earlier exercise): the basis is powers
+    0..deg of the spot standardised by the population std (numpy.std's
+    default, ddof = 0) of the in-the-money spots, and the coefficients are
+    the least-squares solution (numpy.linalg.lstsq)."""
+    s0, k, r, sigma, t, steps, n = 100.0, (104.0 if is_put else 96.0), 0.05, 0.25, 1.0, 10, 4000
+    coeffs, itm, _, _ = orc.lsmc_trace(s0, k, r, sigma, t, steps, n, is_put, deg, 42)
+    paths = orc.gbm(s0, r, sigma, t, steps, n, seed=42)
+    intr = (lambda s: np.maximum(k - s, 0.0)) if is_put else (lambda s: np.maximum(s - k, 0.0))
+    spot = paths[:, steps - 1]
+    sel = intr(spot) > 0
+    assert int(itm[steps - 1]) == int(sel.sum())
+    s = spot[sel]
+    x = (s - s.mean()) / np.std(s)
+    y = intr(paths[sel, steps]) * math.exp(-r * t / steps)
+    want = np.linalg.lstsq(np.vander(x, deg + 1, increasing=True), y, rcond=None)[0]
+    np.testing.assert_allclose(coeffs[steps - 1], want, rtol=1e-9, atol=1e-12)
