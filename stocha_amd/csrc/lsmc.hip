@@ -73,6 +73,34 @@ struct DateMajorHook {
   }
 };
 
+// The same stores through a buffer resource per date column (columns of at
+// most 2^31 - 1 bytes): the column base is wave-uniform (scalar registers)
+// and the lane's byte offset is fixed for the path, so a step's store takes
+// no per-lane address arithmetic; lanes past n have offsets outside the
+// record range and store nothing (no branch around the store).
+template <bool kFp32>
+struct DateMajorBufHook {
+  using T = typename std::conditional<kFp32, float, double>::type;
+  const T* paths;
+  uint32_t n_bytes;  // one column
+  uint32_t off;      // this lane's byte offset in a column
+  uint64_t n;
+  float s0f;
+  template <class M>
+  __device__ __forceinline__ void operator()(uint32_t j, const M& m) const {
+    const __amdgpu_buffer_rsrc_t col = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(paths + (uint64_t)j * n), 0, (int)n_bytes, 0x00020000);
+    if constexpr (kFp32)
+      __builtin_amdgcn_raw_buffer_store_b32(
+          __builtin_bit_cast(uint32_t, s0f * __builtin_amdgcn_exp2f(m.ls * 1.44269504088896341f)),
+          col, (int)off, 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b64(
+          __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned int, m.s), col,
+          (int)off, 0, 0);
+  }
+};
+
 // Block size of the path kernel.  (512-thread FP32 blocks, 8 waves per SIMD
 // and 1,048,576 paths in exactly two rounds of resident threads, measured
 // slower: 168 against 147 us at 1M x 50.)
@@ -81,7 +109,7 @@ constexpr int gen_block() {
   return kLsBlock;
 }
 
-template <bool kFp32>
+template <bool kFp32, bool kBufStore>
 __global__ __launch_bounds__(gen_block<kFp32>()) void lsmc_gen_kernel(
     const GbmConst mc, const StreamDesc sd, const uint64_t n, const uint32_t steps,
     void* __restrict__ paths, double* __restrict__ cf, const double k, const int is_put) {
@@ -101,16 +129,23 @@ __global__ __launch_bounds__(gen_block<kFp32>()) void lsmc_gen_kernel(
   U128 first = stream_at(&sd, (uint64_t)blockIdx.x * kB + threadIdx.x);
   for (uint64_t base = (uint64_t)blockIdx.x * kB; base < n; base += stride) {
     const uint64_t i = base + threadIdx.x;
-    DateMajorHook<kFp32> hook;
-    hook.paths = paths;
+    typename std::conditional<kBufStore, DateMajorBufHook<kFp32>, DateMajorHook<kFp32>>::type hook;
+    hook.paths = static_cast<decltype(hook.paths)>(paths);
     hook.n = n;
-    hook.i = i;
-    hook.valid = i < n;
     hook.s0f = (float)mc.s0;
+    if constexpr (kBufStore) {
+      using T = typename DateMajorBufHook<kFp32>::T;
+      hook.n_bytes = (uint32_t)(n * sizeof(T));
+      // past n: an offset at or beyond the column's bytes
+      hook.off = i < n ? (uint32_t)(i * sizeof(T)) : hook.n_bytes;
+    } else {
+      hook.i = i;
+      hook.valid = i < n;
+    }
     Pcg g{first};
     first = add128(mul128(next.a, first), next.c);
     (void)queue_path<Src, Model, kSlots, kB>(mc, g, sd.inc, steps, &zig, q, false, hook);
-    if (cf != nullptr && hook.valid) {
+    if (cf != nullptr && i < n) {
       double last;
       if constexpr (kFp32)
         last = (double)((const float*)paths)[(uint64_t)(steps - 1) * n + i];
@@ -1021,8 +1056,12 @@ static int lsmc_impl(const GbmConst& gc, const StreamDesc& sd, uint64_t n, uint6
     constexpr int kGb = gen_block<kFp32>();
     const uint64_t gb64 = (n + kGb - 1) / kGb;
     const int gnb = (int)(gb64 > 2048 ? 2048 : gb64);
-    lsmc_gen_kernel<kFp32><<<gnb, kGb, 0, st>>>(gc, sd, n, (uint32_t)steps, paths,
-                                                coop ? nullptr : cf, k, is_put);
+    if (n * esz <= 0x7fffffffull)
+      lsmc_gen_kernel<kFp32, true><<<gnb, kGb, 0, st>>>(gc, sd, n, (uint32_t)steps, paths,
+                                                        coop ? nullptr : cf, k, is_put);
+    else
+      lsmc_gen_kernel<kFp32, false><<<gnb, kGb, 0, st>>>(gc, sd, n, (uint32_t)steps, paths,
+                                                         coop ? nullptr : cf, k, is_put);
   }
   STOCHA_HIP(hipGetLastError());
   if (coop) {
