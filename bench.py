@@ -64,12 +64,12 @@ PEAK_HBM_GBS = 8000.0
 DRAWS_PER_HESTON_STEP = 2 * 1.0296
 # HBM bytes per launch of that kernel: rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (KiB per dispatch) scaled by the same counters on a 256 MiB device copy
-# (FETCH_SIZE reads 131090.375 KiB for 262144 KiB moved -> x1.99972;
-# WRITE_SIZE reads 262144 -> x1.0000): 251.1 KiB x 1.99972 + 4736.16 KiB =
-# 5238.3 KiB.  No path data: the block partials (and the last block's read of
+# (FETCH_SIZE reads 131092.625 KiB for 262144 KiB moved -> x1.99970;
+# WRITE_SIZE reads 262144 -> x1.0000): 251.19 KiB x 1.99970 + 4736.53 KiB =
+# 5238.8 KiB.  No path data: the block partials (and the last block's read of
 # them), the tables, the arguments, and the scratch stores of the two dwords
 # per thread the 7-wave build keeps for its epilogue (~4.4 MB).
-TRAFFIC_BYTES = int((251.125 * 262144.0 / 131090.375 + 4736.15625 * 1.0) * 1024)
+TRAFFIC_BYTES = int((251.1875 * 262144.0 / 131092.625 + 4736.53125 * 1.0) * 1024)
 OPS_PER_STEP_HESTON32 = DRAWS_PER_HESTON_STEP * (25 + 5) + 14
 def time_calls(fn, steps, warmup, S=None):
     """Wall time per call over `steps` calls after `warmup` untimed ones.
