@@ -2,7 +2,8 @@
 GPU through the reference's import path (`import stocha`), so that a user of
 the reference sees its test suite hold on this build:
 tests/test_models.py:31-86 (GBM), :88-135 (Heston), :186-232 (Merton),
-:621-743 (multi_gbm), :745-800 (LSMC).  Sizes, seeds and tolerances are the
+:621-743 (multi_gbm), :745-800 (LSMC); tests/test_greeks.py:131-178, 275-280
+(pathwise Greeks).  Sizes, seeds and tolerances are the
 reference's; the cases are grouped here by property instead of by class.
 """
 import math
@@ -161,3 +162,38 @@ def test_lsmc_reference_cases(stocha):
     # :788-793: the call form
     assert stocha.lsmc_american_option(s0=100.0, k=100.0, r=0.05, sigma=0.20, t=1.0, steps=50,
                                        n_paths=20_000, is_put=False)[0] > 0
+
+
+# ------------------------------------------------------- pathwise Greeks --
+def _bs(s, k, r, sigma, t):
+    d1 = (math.log(s / k) + (r + 0.5 * sigma * sigma) * t) / (sigma * math.sqrt(t))
+    cdf = 0.5 * (1.0 + math.erf(d1 / math.sqrt(2.0)))
+    pdf = math.exp(-0.5 * d1 * d1) / math.sqrt(2.0 * math.pi)
+    return cdf, s * pdf * math.sqrt(t)  # call delta, vega
+
+
+def test_greeks_pathwise_reference_cases(stocha):
+    # tests/test_greeks.py:131-156 (200,000 x 252, seed 42, same tolerances)
+    kw = dict(s0=100.0, r=0.05, sigma=0.2, t=1.0, strike=100.0, n_paths=200_000, n_steps=252,
+              seed=42)
+    delta, vega = _bs(100.0, 100.0, 0.05, 0.2, 1.0)
+    assert abs(stocha.greeks_pathwise(**kw, is_call=True, greeks=["delta"])["delta"]
+               - delta) < 0.015
+    assert abs(stocha.greeks_pathwise(**kw, is_call=False, greeks=["delta"])["delta"]
+               - (delta - 1.0)) < 0.015
+    pw = stocha.greeks_pathwise(**kw, is_call=True, greeks=["vega"])
+    assert abs(pw["vega"] - vega) < 1.5
+    # :169-178: pathwise and bump-and-revalue deltas agree
+    fd = stocha.greeks_fd(model="gbm", params={"s0": 100.0, "r": 0.05, "sigma": 0.2, "t": 1.0},
+                          payoff="call", strike=100.0, n_paths=200_000, n_steps=252,
+                          greeks=["delta"], seed=42)
+    pd = stocha.greeks_pathwise(**kw, is_call=True, greeks=["delta"])
+    assert abs(fd["delta"] - pd["delta"]) < 0.03
+
+
+def test_greeks_pathwise_reference_errors(stocha):
+    base = dict(r=0.05, sigma=0.2, t=1.0, strike=100.0, is_call=True, n_paths=1000, n_steps=10)
+    with pytest.raises(ValueError, match="only supports"):  # tests/test_greeks.py:158-164
+        stocha.greeks_pathwise(s0=100.0, **base, greeks=["gamma"])
+    with pytest.raises(ValueError, match="s0 must be positive"):  # :275-280
+        stocha.greeks_pathwise(s0=-1.0, **base, greeks=["delta"])
